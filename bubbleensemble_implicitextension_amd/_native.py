@@ -1,0 +1,101 @@
+"""Loader for libbex.so, the gfx950 solver library (C ABI in include/bex_capi.h).
+
+The library is built in-tree by ``python -m bubbleensemble_implicitextension_amd.build``.
+There is no fallback: if the library is missing or cannot be loaded, every solver entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbex.so")
+
+NSTAT = 32
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("NS", "UPS", "UD", "TPB", "SPB", "NC", "CBW", "CCI", "NUP", "NSP", "NGP", "NiGP",
+                 "NUA", "NiUA", "NSA", "NiSA", "NE", "NDO", "Algorithm")]
+
+
+_lib = None
+
+_SIGNATURES = {
+    "bex_compiled_dims": ([ctypes.POINTER(ProblemDesc)], ctypes.c_int),
+    "bex_create": ([ctypes.POINTER(ProblemDesc), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+                   ctypes.c_int),
+    "bex_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_solver_option": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "bex_solver_option_indexed": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double],
+                                  ctypes.c_int),
+    "bex_set_host_unit": ([ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_double], ctypes.c_int),
+    "bex_set_host_system": ([ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_double],
+                            ctypes.c_int),
+    "bex_set_host_global": ([ctypes.c_void_p] + [ctypes.c_int] * 2 + [ctypes.c_double],
+                            ctypes.c_int),
+    "bex_set_host_coupling": ([ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_double],
+                              ctypes.c_int),
+    "bex_set_host_dense_index": ([ctypes.c_void_p] + [ctypes.c_int] * 2 + [ctypes.c_double],
+                                 ctypes.c_int),
+    "bex_set_host_dense_state": ([ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_double],
+                                 ctypes.c_int),
+    "bex_get_host_unit": ([ctypes.c_void_p] + [ctypes.c_int] * 4 +
+                          [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_get_host_system": ([ctypes.c_void_p] + [ctypes.c_int] * 3 +
+                            [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_get_host_global": ([ctypes.c_void_p] + [ctypes.c_int] * 2 +
+                            [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_get_host_coupling": ([ctypes.c_void_p] + [ctypes.c_int] * 4 +
+                              [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_get_host_dense_index": ([ctypes.c_void_p] + [ctypes.c_int] * 2 +
+                                 [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_get_host_dense_state": ([ctypes.c_void_p] + [ctypes.c_int] * 5 +
+                                 [ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_host_view": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                       ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)],
+                      ctypes.c_int),
+    "bex_sync_h2d": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_sync_d2h": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_solve": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_synchronise_device": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_insert_synchronisation_point": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_synchronise_solver": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "bex_reset_stats": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def load():
+    """Returns the loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m bubbleensemble_implicitextension_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+class BexError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+def check(code, handle=None):
+    if code != 0:
+        msg = load().bex_last_error(handle)
+        raise BexError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,788 @@
+// Host side of libbex.so: the C ABI declared in include/bex_capi.h.
+//
+// A bex_solver is the MI355X counterpart of one reference ProblemSolver object
+// (SourceCodes/CoupledSystems_PerBlock_Interface.cuh): it owns pinned host buffers and device
+// buffers in the reference's structure-of-arrays layout, an in-order HIP stream and an event,
+// and launches the solver kernel of bex_solver_device.h once per Solve().
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/bex_capi.h"
+#include "bex_solver_device.h"
+
+namespace {
+
+std::string g_create_error;
+
+struct Buf {
+  void* h = nullptr;
+  void* d = nullptr;
+  long long n = 0;
+  int esz = 8;
+  size_t bytes() const { return (size_t)(n > 0 ? n : 1) * esz; }
+};
+
+}  // namespace
+
+struct bex_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t event = nullptr;
+  bex_problem_desc d{};
+  // Thread configuration of the reference layout (Interface.cuh:386-391).
+  int LogicalThreadsPerBlock = 0, ThreadPaddingPerBlock = 0, GridSize = 0;
+  long long TLT = 0;
+  long long CompactCouplingSize = 0;  // per coupling, reference storage (Interface.cuh:428-438)
+
+  Buf TimeDomain, ActualState, ActualTime, UnitParameters, SystemParameters, GlobalParameters,
+      IntegerGlobalParameters, UnitAccessories, IntegerUnitAccessories, SystemAccessories,
+      IntegerSystemAccessories, CouplingMatrix, CouplingStrength, CouplingIndex,
+      DenseOutputIndex, DenseOutputTimeInstances, DenseOutputStates;
+  double* h_CouplingOriginal = nullptr;  // full [c][N][N], host only (Interface.cuh:1076)
+  double* d_CouplingFull = nullptr;      // full [c][N][N] expanded from the compact storage
+  double* h_tol = nullptr;               // [2*UD + NE] rtol, atol, event tolerance
+  int* h_dir = nullptr;                  // [NE]
+  double* d_tol = nullptr;
+  int* d_dir = nullptr;
+  long long* d_stats = nullptr;
+
+  double InitialTimeStep = 1e-2, MaximumTimeStep = 1.0e6, MinimumTimeStep = 1.0e-12;
+  double TimeStepGrowLimit = 5.0, TimeStepShrinkLimit = 0.1, DenseOutputMinimumTimeStep = 0.0;
+  int ActiveSystems = 0, DenseOutputSaveFrequency = 1;
+  double BiCGRhoTolerance = 1e-6, BiCGResidualTolerance = 1e-6;
+  int BiCGMaxIterations = 19;
+  bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
+  std::string err;
+};
+
+namespace {
+
+int fail(bex_solver* s, int code, const std::string& msg) {
+  if (s)
+    s->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define BEX_HIP(s, call)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail((s), BEX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int bound(bex_solver* s, const char* fn, const char* what, long long v, long long lo, long long hi) {
+  if (v < lo || v > hi) {
+    char m[256];
+    if (lo > hi)
+      snprintf(m, sizeof m, "%s: %s index %lld out of range (acceptable index: none)", fn, what, v);
+    else
+      snprintf(m, sizeof m, "%s: %s index %lld out of range (acceptable index: %lld-%lld)", fn,
+               what, v, lo, hi);
+    return fail(s, BEX_E_BOUND, m);
+  }
+  return BEX_OK;
+}
+
+using KernelFn = void (*)(BexKernelArgs);
+
+template <int N>
+KernelFn pick(int alg) {
+  return alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<N, bexk::kRK4>
+                        : (KernelFn)bexk::solve_kernel<N, bexk::kRKCK45>;
+}
+
+bool kernel_for(int ups, int alg, KernelFn* fn, size_t* lds, int* spb) {
+  switch (ups) {
+    case 8: *fn = pick<8>(alg); *lds = bexk::lds_bytes<8>(); *spb = bexk::Geometry<8>::SPB; return true;
+    case 16: *fn = pick<16>(alg); *lds = bexk::lds_bytes<16>(); *spb = bexk::Geometry<16>::SPB; return true;
+    case 32: *fn = pick<32>(alg); *lds = bexk::lds_bytes<32>(); *spb = bexk::Geometry<32>::SPB; return true;
+    case 64: *fn = pick<64>(alg); *lds = bexk::lds_bytes<64>(); *spb = bexk::Geometry<64>::SPB; return true;
+    default: return false;
+  }
+}
+
+// Logical thread index of (sid, uid) in the reference layout (Interface.cuh:875-879).
+long long unit_index(const bex_solver* s, int sid, int uid) {
+  int block = sid / s->d.SPB, lsid = sid % s->d.SPB;
+  return (long long)block * (s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock) +
+         (long long)lsid * s->d.UPS + uid;
+}
+
+Buf* buffer_of(bex_solver* s, int var) {
+  switch (var) {
+    case BEX_TimeDomain: return &s->TimeDomain;
+    case BEX_ActualTime: return &s->ActualTime;
+    case BEX_ActualState: return &s->ActualState;
+    case BEX_UnitParameters: return &s->UnitParameters;
+    case BEX_SystemParameters: return &s->SystemParameters;
+    case BEX_GlobalParameters: return &s->GlobalParameters;
+    case BEX_IntegerGlobalParameters: return &s->IntegerGlobalParameters;
+    case BEX_UnitAccessories: return &s->UnitAccessories;
+    case BEX_IntegerUnitAccessories: return &s->IntegerUnitAccessories;
+    case BEX_SystemAccessories: return &s->SystemAccessories;
+    case BEX_IntegerSystemAccessories: return &s->IntegerSystemAccessories;
+    case BEX_CouplingMatrix: return &s->CouplingMatrix;
+    case BEX_CouplingStrength: return &s->CouplingStrength;
+    case BEX_CouplingIndex: return &s->CouplingIndex;
+    case BEX_DenseIndex: return &s->DenseOutputIndex;
+    case BEX_DenseTime: return &s->DenseOutputTimeInstances;
+    case BEX_DenseState: return &s->DenseOutputStates;
+    default: return nullptr;
+  }
+}
+
+void put(Buf& b, long long i, double v) {
+  if (b.esz == 8)
+    static_cast<double*>(b.h)[i] = v;
+  else
+    static_cast<int*>(b.h)[i] = (int)v;
+}
+double get(const Buf& b, long long i) {
+  return b.esz == 8 ? static_cast<const double*>(b.h)[i] : (double)static_cast<const int*>(b.h)[i];
+}
+
+// Compact index of coupling element (row, col) for the reference's four storage schemes
+// (Interface.cuh:1001-1070); returns -1 outside the stored band.
+long long compact_index(const bex_solver* s, int c, int row, int col) {
+  const int N = s->d.UPS, CBW = s->d.CBW, CCI = s->d.CCI;
+  if (CCI == 0 && CBW == 0) return row + (long long)col * N + (long long)c * N * N;
+  int mod = col - row + CBW;
+  if (mod >= N) mod -= N;
+  if (mod < 0) mod += N;
+  int lim = (CBW > 0) ? 2 * CBW + 1 : N;
+  if (mod < 0 || mod >= lim) return -1;
+  if (CCI == 0) return row + (long long)mod * N + (long long)c * N * lim;
+  return mod + (long long)c * lim;
+}
+
+int copy_h2d(bex_solver* s, Buf& b) {
+  if (b.n > 0) BEX_HIP(s, hipMemcpyAsync(b.d, b.h, (size_t)b.n * b.esz, hipMemcpyHostToDevice, s->stream));
+  return BEX_OK;
+}
+int copy_d2h(bex_solver* s, Buf& b) {
+  if (b.n > 0) BEX_HIP(s, hipMemcpyAsync(b.h, b.d, (size_t)b.n * b.esz, hipMemcpyDeviceToHost, s->stream));
+  return BEX_OK;
+}
+
+// The kernel reads a full N x N matrix per coupling; expand the compact storage into it.
+int upload_coupling(bex_solver* s) {
+  const int N = s->d.UPS, NC = s->d.NC;
+  double* full = nullptr;
+  BEX_HIP(s, hipHostMalloc((void**)&full, sizeof(double) * (size_t)NC * N * N, hipHostMallocDefault));
+  for (int c = 0; c < NC; c++)
+    for (int col = 0; col < N; col++)
+      for (int row = 0; row < N; row++) {
+        long long k = compact_index(s, c, row, col);
+        full[row + (long long)col * N + (long long)c * N * N] =
+            (k >= 0) ? static_cast<double*>(s->CouplingMatrix.h)[k] : 0.0;
+      }
+  hipError_t e = hipMemcpyAsync(s->d_CouplingFull, full, sizeof(double) * (size_t)NC * N * N,
+                                hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  hipHostFree(full);
+  if (e != hipSuccess) return fail(s, BEX_E_HIP, std::string("coupling upload: ") + hipGetErrorString(e));
+  return BEX_OK;
+}
+
+int upload_tolerances(bex_solver* s) {
+  const int n = 2 * s->d.UD + (s->d.NE > 0 ? s->d.NE : 1);
+  BEX_HIP(s, hipMemcpyAsync(s->d_tol, s->h_tol, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+  BEX_HIP(s, hipMemcpyAsync(s->d_dir, s->h_dir, sizeof(int) * (s->d.NE > 0 ? s->d.NE : 1),
+                            hipMemcpyHostToDevice, s->stream));
+  // The staging arrays are reused by later calls: finish the copy before returning.
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  return BEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bex_compiled_dims(bex_problem_desc* out) {
+  if (!out) return BEX_E_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  using D = BexSystemDims;
+  out->UD = D::UD;
+  out->NUP = D::NUP;
+  out->NSP = D::NSP;
+  out->NGP = D::NGP;
+  out->NiGP = D::NiGP;
+  out->NUA = D::NUA;
+  out->NiUA = D::NiUA;
+  out->NSA = D::NSA;
+  out->NiSA = D::NiSA;
+  out->NE = D::NE;
+  out->NC = D::NC;
+  return BEX_OK;
+}
+
+const char* bex_last_error(const bex_solver* s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
+  if (!desc || !out) return fail(nullptr, BEX_E_ARGUMENT, "bex_create: null argument");
+  *out = nullptr;
+  const bex_problem_desc& d = *desc;
+  using D = BexSystemDims;
+  if (d.UD != D::UD || d.NUP != D::NUP || d.NSP != D::NSP || d.NGP != D::NGP ||
+      d.NiGP != D::NiGP || d.NUA != D::NUA || d.NiUA != D::NiUA || d.NSA != D::NSA ||
+      d.NiSA != D::NiSA || d.NE != D::NE) {
+    char m[512];
+    snprintf(m, sizeof m,
+             "bex_create: dimensions (UD %d NUP %d NSP %d NGP %d NiGP %d NUA %d NiUA %d NSA %d "
+             "NiSA %d NE %d) differ from the compiled system definition (UD %d NUP %d NSP %d "
+             "NGP %d NiGP %d NUA %d NiUA %d NSA %d NiSA %d NE %d)",
+             d.UD, d.NUP, d.NSP, d.NGP, d.NiGP, d.NUA, d.NiUA, d.NSA, d.NiSA, d.NE, D::UD,
+             D::NUP, D::NSP, D::NGP, D::NiGP, D::NUA, D::NiUA, D::NSA, D::NiSA, D::NE);
+    return fail(nullptr, BEX_E_MISMATCH, m);
+  }
+  if (d.NS <= 0 || d.TPB <= 0 || d.SPB <= 0 || d.NC <= 0 || d.NDO < 0 || d.CBW < 0 ||
+      (d.CCI != 0 && d.CCI != 1) || (d.Algorithm != BEX_RK4 && d.Algorithm != BEX_RKCK45))
+    return fail(nullptr, BEX_E_ARGUMENT, "bex_create: invalid problem description");
+  KernelFn fn;
+  size_t lds;
+  int spb;
+  if (!kernel_for(d.UPS, d.Algorithm, &fn, &lds, &spb))
+    return fail(nullptr, BEX_E_UNSUPPORTED,
+                "bex_create: units per system must be 8, 16, 32 or 64 in this build");
+
+  bex_solver* s = new bex_solver();
+  s->d = d;
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&s->event);
+  if (e != hipSuccess) {
+    g_create_error = std::string("bex_create: ") + hipGetErrorString(e);
+    delete s;
+    return BEX_E_HIP;
+  }
+
+  // Thread configuration (Interface.cuh:386-391).
+  s->LogicalThreadsPerBlock = d.SPB * d.UPS;
+  int launches = s->LogicalThreadsPerBlock / d.TPB + (s->LogicalThreadsPerBlock % d.TPB ? 1 : 0);
+  s->ThreadPaddingPerBlock = launches * d.TPB - s->LogicalThreadsPerBlock;
+  s->GridSize = d.NS / d.SPB + (d.NS % d.SPB ? 1 : 0);
+  s->TLT = (long long)(s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock) * s->GridSize;
+  s->ActiveSystems = d.NS;
+
+  // Sizes (Interface.cuh:408-438).
+  const int N = d.UPS;
+  if (d.CCI == 0 && d.CBW == 0) s->CompactCouplingSize = (long long)N * N;
+  else if (d.CCI == 0) s->CompactCouplingSize = (long long)N * (2 * d.CBW + 1);
+  else if (d.CBW > 0) s->CompactCouplingSize = 2 * d.CBW + 1;
+  else s->CompactCouplingSize = N;
+  struct Spec { Buf* b; long long n; int esz; };
+  Spec specs[] = {
+      {&s->TimeDomain, 2LL * d.NS, 8},
+      {&s->ActualState, s->TLT * d.UD, 8},
+      {&s->ActualTime, d.NS, 8},
+      {&s->UnitParameters, s->TLT * d.NUP, 8},
+      {&s->SystemParameters, (long long)d.NS * d.NSP, 8},
+      {&s->GlobalParameters, d.NGP, 8},
+      {&s->IntegerGlobalParameters, d.NiGP, 4},
+      {&s->UnitAccessories, s->TLT * d.NUA, 8},
+      {&s->IntegerUnitAccessories, s->TLT * d.NiUA, 4},
+      {&s->SystemAccessories, (long long)d.NS * d.NSA, 8},
+      {&s->IntegerSystemAccessories, (long long)d.NS * d.NiSA, 4},
+      {&s->CouplingMatrix, (long long)d.NC * s->CompactCouplingSize, 8},
+      {&s->CouplingStrength, (long long)d.NC * d.NS, 8},
+      {&s->CouplingIndex, d.NC, 4},
+      {&s->DenseOutputIndex, d.NS, 4},
+      {&s->DenseOutputTimeInstances, (long long)d.NS * d.NDO, 8},
+      {&s->DenseOutputStates, s->TLT * d.UD * d.NDO, 8},
+  };
+  size_t required = 0;
+  for (auto& sp : specs) {
+    sp.b->n = sp.n;
+    sp.b->esz = sp.esz;
+    required += sp.b->bytes();
+  }
+  required += sizeof(double) * (size_t)d.NC * N * N + sizeof(long long) * (size_t)d.NS * BEX_NSTAT;
+  size_t freeMem = 0, totalMem = 0;
+  if (hipMemGetInfo(&freeMem, &totalMem) == hipSuccess && required >= freeMem) {
+    g_create_error = "bex_create: the required amount of global memory is larger than the free";
+    bex_destroy(s);
+    return BEX_E_MEMORY;
+  }
+  bool ok = true;
+  for (auto& sp : specs) {
+    if (hipHostMalloc(&sp.b->h, sp.b->bytes(), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&sp.b->d, sp.b->bytes()) != hipSuccess) {
+      ok = false;
+      break;
+    }
+    std::memset(sp.b->h, 0, sp.b->bytes());
+  }
+  const int ntol = 2 * d.UD + (d.NE > 0 ? d.NE : 1);
+  ok = ok && hipHostMalloc((void**)&s->h_CouplingOriginal, sizeof(double) * (size_t)d.NC * N * N,
+                           hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_CouplingFull, sizeof(double) * (size_t)d.NC * N * N) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&s->h_tol, sizeof(double) * ntol, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&s->h_dir, sizeof(int) * (d.NE > 0 ? d.NE : 1), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_tol, sizeof(double) * ntol) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_dir, sizeof(int) * (d.NE > 0 ? d.NE : 1)) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_stats, sizeof(long long) * (size_t)d.NS * BEX_NSTAT) == hipSuccess;
+  if (!ok) {
+    g_create_error = "bex_create: memory allocation failed";
+    bex_destroy(s);
+    return BEX_E_MEMORY;
+  }
+  std::memset(s->h_CouplingOriginal, 0, sizeof(double) * (size_t)d.NC * N * N);
+  // Default options (Interface.cuh:613-635).
+  for (int i = 0; i < d.UD; i++) {
+    s->h_tol[i] = 1e-8;
+    s->h_tol[d.UD + i] = 1e-8;
+  }
+  for (int i = 0; i < (d.NE > 0 ? d.NE : 1); i++) {
+    s->h_tol[2 * d.UD + i] = 1e-6;
+    s->h_dir[i] = 0;
+  }
+  int rc = upload_tolerances(s);
+  if (rc == BEX_OK) rc = upload_coupling(s);
+  if (rc == BEX_OK && hipMemsetAsync(s->d_stats, 0, sizeof(long long) * (size_t)d.NS * BEX_NSTAT,
+                                     s->stream) != hipSuccess)
+    rc = BEX_E_HIP;
+  if (rc == BEX_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = BEX_E_HIP;
+  if (rc != BEX_OK) {
+    g_create_error = s->err;
+    bex_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return BEX_OK;
+}
+
+int bex_destroy(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  hipSetDevice(s->device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  Buf* bufs[] = {&s->TimeDomain, &s->ActualState, &s->ActualTime, &s->UnitParameters,
+                 &s->SystemParameters, &s->GlobalParameters, &s->IntegerGlobalParameters,
+                 &s->UnitAccessories, &s->IntegerUnitAccessories, &s->SystemAccessories,
+                 &s->IntegerSystemAccessories, &s->CouplingMatrix, &s->CouplingStrength,
+                 &s->CouplingIndex, &s->DenseOutputIndex, &s->DenseOutputTimeInstances,
+                 &s->DenseOutputStates};
+  for (Buf* b : bufs) {
+    if (b->h) hipHostFree(b->h);
+    if (b->d) hipFree(b->d);
+  }
+  if (s->h_CouplingOriginal) hipHostFree(s->h_CouplingOriginal);
+  if (s->d_CouplingFull) hipFree(s->d_CouplingFull);
+  if (s->h_tol) hipHostFree(s->h_tol);
+  if (s->h_dir) hipHostFree(s->h_dir);
+  if (s->d_tol) hipFree(s->d_tol);
+  if (s->d_dir) hipFree(s->d_dir);
+  if (s->d_stats) hipFree(s->d_stats);
+  if (s->event) hipEventDestroy(s->event);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return BEX_OK;
+}
+
+int bex_solver_option(bex_solver* s, int option, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  switch (option) {
+    case BEX_InitialTimeStep: s->InitialTimeStep = v; break;
+    case BEX_ActiveSystems: s->ActiveSystems = (int)v; break;
+    case BEX_MaximumTimeStep: s->MaximumTimeStep = v; break;
+    case BEX_MinimumTimeStep: s->MinimumTimeStep = v; break;
+    case BEX_TimeStepGrowLimit: s->TimeStepGrowLimit = v; break;
+    case BEX_TimeStepShrinkLimit: s->TimeStepShrinkLimit = v; break;
+    case BEX_DenseOutputMinimumTimeStep: s->DenseOutputMinimumTimeStep = v; break;
+    case BEX_DenseOutputSaveFrequency: s->DenseOutputSaveFrequency = (int)v; break;
+    case BEX_SharedGlobalVariables: s->SharedGlobalVariables = (v != 0.0); break;
+    case BEX_SharedCouplingMatrices: s->SharedCouplingMatrices = (v != 0.0); break;
+    case BEX_BiCGRhoTolerance: s->BiCGRhoTolerance = v; break;
+    case BEX_BiCGResidualTolerance: s->BiCGResidualTolerance = v; break;
+    case BEX_BiCGMaxIterations: s->BiCGMaxIterations = (int)v; break;
+    default:
+      return fail(s, BEX_E_OPTION,
+                  "SolverOption: this option needs 2 input arguments or is not applicable");
+  }
+  return BEX_OK;
+}
+
+int bex_solver_option_indexed(bex_solver* s, int option, int index, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  const int UD = s->d.UD, NE = s->d.NE;
+  int rc;
+  switch (option) {
+    case BEX_RelativeTolerance:
+      if ((rc = bound(s, "SolverOption", "RelativeTolerance", index, 0, UD - 1))) return rc;
+      s->h_tol[index] = v;
+      break;
+    case BEX_AbsoluteTolerance:
+      if ((rc = bound(s, "SolverOption", "AbsoluteTolerance", index, 0, UD - 1))) return rc;
+      s->h_tol[UD + index] = v;
+      break;
+    case BEX_EventTolerance:
+      if ((rc = bound(s, "SolverOption", "EventTolerance", index, 0, NE - 1))) return rc;
+      s->h_tol[2 * UD + index] = v;
+      break;
+    case BEX_EventDirection:
+      if ((rc = bound(s, "SolverOption", "EventDirection", index, 0, NE - 1))) return rc;
+      s->h_dir[index] = (int)v;
+      break;
+    default:
+      return fail(s, BEX_E_OPTION,
+                  "SolverOption: this option needs 1 input argument or is not applicable");
+  }
+  return upload_tolerances(s);
+}
+
+int bex_set_host_unit(bex_solver* s, int sid, int uid, int var, int k, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "SetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  if ((rc = bound(s, "SetHost", "UnitNumber", uid, 0, s->d.UPS - 1))) return rc;
+  long long g = unit_index(s, sid, uid);
+  int lim;
+  Buf* b;
+  switch (var) {
+    case BEX_ActualState: lim = s->d.UD; b = &s->ActualState; break;
+    case BEX_UnitParameters: lim = s->d.NUP; b = &s->UnitParameters; break;
+    case BEX_UnitAccessories: lim = s->d.NUA; b = &s->UnitAccessories; break;
+    case BEX_IntegerUnitAccessories: lim = s->d.NiUA; b = &s->IntegerUnitAccessories; break;
+    default: return fail(s, BEX_E_VARIABLE, "SetHost: this option needs different argument configuration or not applicable");
+  }
+  if ((rc = bound(s, "SetHost", "SerialNumber", k, 0, lim - 1))) return rc;
+  put(*b, g + (long long)k * s->TLT, v);
+  return BEX_OK;
+}
+
+int bex_get_host_unit(bex_solver* s, int sid, int uid, int var, int k, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "GetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  if ((rc = bound(s, "GetHost", "UnitNumber", uid, 0, s->d.UPS - 1))) return rc;
+  long long g = unit_index(s, sid, uid);
+  int lim;
+  Buf* b;
+  switch (var) {
+    case BEX_ActualState: lim = s->d.UD; b = &s->ActualState; break;
+    case BEX_UnitParameters: lim = s->d.NUP; b = &s->UnitParameters; break;
+    case BEX_UnitAccessories: lim = s->d.NUA; b = &s->UnitAccessories; break;
+    case BEX_IntegerUnitAccessories: lim = s->d.NiUA; b = &s->IntegerUnitAccessories; break;
+    default: return fail(s, BEX_E_VARIABLE, "GetHost: this option needs different argument configuration or not applicable");
+  }
+  if ((rc = bound(s, "GetHost", "SerialNumber", k, 0, lim - 1))) return rc;
+  *v = get(*b, g + (long long)k * s->TLT);
+  return BEX_OK;
+}
+
+static int system_slot(bex_solver* s, const char* fn, int var, int k, Buf** b) {
+  int lim;
+  switch (var) {
+    case BEX_TimeDomain: lim = 2; *b = &s->TimeDomain; break;
+    case BEX_ActualTime: lim = 1; *b = &s->ActualTime; break;
+    case BEX_SystemParameters: lim = s->d.NSP; *b = &s->SystemParameters; break;
+    case BEX_SystemAccessories: lim = s->d.NSA; *b = &s->SystemAccessories; break;
+    case BEX_IntegerSystemAccessories: lim = s->d.NiSA; *b = &s->IntegerSystemAccessories; break;
+    case BEX_DenseTime: lim = s->d.NDO; *b = &s->DenseOutputTimeInstances; break;
+    case BEX_CouplingStrength: lim = s->d.NC; *b = &s->CouplingStrength; break;
+    default: return fail(s, BEX_E_VARIABLE, std::string(fn) + ": this option needs different argument configuration or not applicable");
+  }
+  return bound(s, fn, "SerialNumber", k, 0, lim - 1);
+}
+
+int bex_set_host_system(bex_solver* s, int sid, int var, int k, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "SetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  Buf* b;
+  if ((rc = system_slot(s, "SetHost", var, k, &b))) return rc;
+  put(*b, sid + (long long)k * s->d.NS, v);
+  return BEX_OK;
+}
+
+int bex_get_host_system(bex_solver* s, int sid, int var, int k, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "GetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  Buf* b;
+  if ((rc = system_slot(s, "GetHost", var, k, &b))) return rc;
+  *v = get(*b, sid + (long long)k * s->d.NS);
+  return BEX_OK;
+}
+
+static int global_slot(bex_solver* s, const char* fn, int var, int k, Buf** b) {
+  int lim;
+  switch (var) {
+    case BEX_GlobalParameters: lim = s->d.NGP; *b = &s->GlobalParameters; break;
+    case BEX_IntegerGlobalParameters: lim = s->d.NiGP; *b = &s->IntegerGlobalParameters; break;
+    case BEX_CouplingIndex: lim = s->d.NC; *b = &s->CouplingIndex; break;
+    default: return fail(s, BEX_E_VARIABLE, std::string(fn) + ": this option needs different argument configuration or not applicable");
+  }
+  return bound(s, fn, "SerialNumber", k, 0, lim - 1);
+}
+
+int bex_set_host_global(bex_solver* s, int var, int k, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  Buf* b;
+  int rc;
+  if ((rc = global_slot(s, "SetHost", var, k, &b))) return rc;
+  put(*b, k, v);
+  return BEX_OK;
+}
+
+int bex_get_host_global(bex_solver* s, int var, int k, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  Buf* b;
+  int rc;
+  if ((rc = global_slot(s, "GetHost", var, k, &b))) return rc;
+  *v = get(*b, k);
+  return BEX_OK;
+}
+
+int bex_set_host_coupling(bex_solver* s, int c, int var, int row, int col, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  int rc;
+  const int N = s->d.UPS;
+  if ((rc = bound(s, "SetHost", "CouplingNumber", c, 0, s->d.NC - 1))) return rc;
+  if ((rc = bound(s, "SetHost", "Coupling Matrix Row", row, 0, N - 1))) return rc;
+  if ((rc = bound(s, "SetHost", "Coupling Matrix Col", col, 0, N - 1))) return rc;
+  if (var != BEX_CouplingMatrix)
+    return fail(s, BEX_E_VARIABLE, "SetHost: this option needs different argument configuration or not applicable");
+  long long k = compact_index(s, c, row, col);
+  if (k < 0) {
+    int lim = (s->d.CBW > 0) ? 2 * s->d.CBW + 1 : N;
+    int mod = ((col - row + s->d.CBW) % N + N) % N;
+    return bound(s, "SetHost", "Coupling Matrix Shifted Diagonal", mod, 0, lim - 1);
+  }
+  s->h_CouplingOriginal[row + (long long)col * N + (long long)c * N * N] = v;
+  static_cast<double*>(s->CouplingMatrix.h)[k] = v;
+  return BEX_OK;
+}
+
+int bex_get_host_coupling(bex_solver* s, int c, int var, int row, int col, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  int rc;
+  const int N = s->d.UPS;
+  if ((rc = bound(s, "GetHost", "CouplingNumber", c, 0, s->d.NC - 1))) return rc;
+  if ((rc = bound(s, "GetHost", "Coupling Matrix Row", row, 0, N - 1))) return rc;
+  if ((rc = bound(s, "GetHost", "Coupling Matrix Col", col, 0, N - 1))) return rc;
+  if (var != BEX_CouplingMatrix)
+    return fail(s, BEX_E_VARIABLE, "GetHost: this option needs different argument configuration or not applicable");
+  *v = s->h_CouplingOriginal[row + (long long)col * N + (long long)c * N * N];  // Interface.cuh:1469
+  return BEX_OK;
+}
+
+int bex_set_host_dense_index(bex_solver* s, int sid, int var, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "SetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  if (var != BEX_DenseIndex)
+    return fail(s, BEX_E_VARIABLE, "SetHost: this option needs different argument configuration or not applicable");
+  static_cast<int*>(s->DenseOutputIndex.h)[sid] = (int)v;
+  return BEX_OK;
+}
+
+int bex_get_host_dense_index(bex_solver* s, int sid, int var, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  int rc;
+  if ((rc = bound(s, "GetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  if (var != BEX_DenseIndex)
+    return fail(s, BEX_E_VARIABLE, "GetHost: this option needs different argument configuration or not applicable");
+  *v = static_cast<int*>(s->DenseOutputIndex.h)[sid];
+  return BEX_OK;
+}
+
+static int dense_state_slot(bex_solver* s, const char* fn, int sid, int uid, int var, int comp,
+                            int k, long long* idx) {
+  int rc;
+  if ((rc = bound(s, fn, "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
+  if ((rc = bound(s, fn, "UnitNumber", uid, 0, s->d.UPS - 1))) return rc;
+  if ((rc = bound(s, fn, "ComponentNumber in dense state", comp, 0, s->d.UD - 1))) return rc;
+  if (var != BEX_DenseState)
+    return fail(s, BEX_E_VARIABLE, std::string(fn) + ": this option needs different argument configuration or not applicable");
+  if ((rc = bound(s, fn, "DenseState", k, 0, s->d.NDO - 1))) return rc;
+  *idx = unit_index(s, sid, uid) + (long long)comp * s->TLT + (long long)k * s->TLT * s->d.UD;
+  return BEX_OK;
+}
+
+int bex_set_host_dense_state(bex_solver* s, int sid, int uid, int var, int comp, int k, double v) {
+  if (!s) return BEX_E_ARGUMENT;
+  long long i;
+  int rc;
+  if ((rc = dense_state_slot(s, "SetHost", sid, uid, var, comp, k, &i))) return rc;
+  static_cast<double*>(s->DenseOutputStates.h)[i] = v;
+  return BEX_OK;
+}
+
+int bex_get_host_dense_state(bex_solver* s, int sid, int uid, int var, int comp, int k, double* v) {
+  if (!s || !v) return BEX_E_ARGUMENT;
+  long long i;
+  int rc;
+  if ((rc = dense_state_slot(s, "GetHost", sid, uid, var, comp, k, &i))) return rc;
+  *v = static_cast<double*>(s->DenseOutputStates.h)[i];
+  return BEX_OK;
+}
+
+int bex_host_view(bex_solver* s, int var, void** ptr, long long* count, int* elem_bytes) {
+  if (!s || !ptr || !count || !elem_bytes) return BEX_E_ARGUMENT;
+  Buf* b = buffer_of(s, var);
+  if (!b) return fail(s, BEX_E_VARIABLE, "bex_host_view: no single buffer for this variable");
+  *ptr = b->h;
+  *count = b->n;
+  *elem_bytes = b->esz;
+  return BEX_OK;
+}
+
+int bex_sync_h2d(bex_solver* s, int var) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  int rc = BEX_OK;
+  switch (var) {
+    case BEX_CouplingMatrix: return upload_coupling(s);
+    case BEX_DenseOutput:
+      if ((rc = copy_h2d(s, s->DenseOutputIndex))) return rc;
+      if ((rc = copy_h2d(s, s->DenseOutputTimeInstances))) return rc;
+      return copy_h2d(s, s->DenseOutputStates);
+    case BEX_All: {
+      Buf* all[] = {&s->TimeDomain, &s->ActualState, &s->ActualTime, &s->UnitParameters,
+                    &s->SystemParameters, &s->GlobalParameters, &s->IntegerGlobalParameters,
+                    &s->UnitAccessories, &s->IntegerUnitAccessories, &s->SystemAccessories,
+                    &s->IntegerSystemAccessories, &s->CouplingStrength, &s->CouplingIndex,
+                    &s->DenseOutputIndex, &s->DenseOutputTimeInstances, &s->DenseOutputStates};
+      for (Buf* b : all)
+        if ((rc = copy_h2d(s, *b))) return rc;
+      return upload_coupling(s);
+    }
+    default: {
+      Buf* b = buffer_of(s, var);
+      if (!b || var == BEX_DenseIndex || var == BEX_DenseTime || var == BEX_DenseState)
+        return fail(s, BEX_E_VARIABLE, "SynchroniseFromHostToDevice: this option is not applicable");
+      return copy_h2d(s, *b);
+    }
+  }
+}
+
+int bex_sync_d2h(bex_solver* s, int var) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  int rc = BEX_OK;
+  switch (var) {
+    case BEX_CouplingMatrix: return BEX_OK;  // the kernel never writes the coupling matrix
+    case BEX_DenseOutput:
+      if ((rc = copy_d2h(s, s->DenseOutputIndex))) return rc;
+      if ((rc = copy_d2h(s, s->DenseOutputTimeInstances))) return rc;
+      return copy_d2h(s, s->DenseOutputStates);
+    case BEX_All: {
+      Buf* all[] = {&s->TimeDomain, &s->ActualState, &s->ActualTime, &s->UnitParameters,
+                    &s->SystemParameters, &s->GlobalParameters, &s->IntegerGlobalParameters,
+                    &s->UnitAccessories, &s->IntegerUnitAccessories, &s->SystemAccessories,
+                    &s->IntegerSystemAccessories, &s->CouplingStrength, &s->CouplingIndex,
+                    &s->DenseOutputIndex, &s->DenseOutputTimeInstances, &s->DenseOutputStates};
+      for (Buf* b : all)
+        if ((rc = copy_d2h(s, *b))) return rc;
+      return BEX_OK;
+    }
+    default: {
+      Buf* b = buffer_of(s, var);
+      if (!b || var == BEX_DenseIndex || var == BEX_DenseTime || var == BEX_DenseState)
+        return fail(s, BEX_E_VARIABLE, "SynchroniseFromDeviceToHost: this option is not applicable");
+      return copy_d2h(s, *b);
+    }
+  }
+}
+
+int bex_solve(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  KernelFn fn;
+  size_t lds;
+  int spb;
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, &fn, &lds, &spb))
+    return fail(s, BEX_E_UNSUPPORTED, "Solve: unsupported units per system");
+  const int ND = s->d.UD;
+  BexKernelArgs a{};
+  a.NS = s->d.NS;
+  a.ActiveSystems = s->ActiveSystems;
+  a.NDO = s->d.NDO;
+  a.first_system = 0;
+  a.TLT = s->TLT;
+  a.TimeDomain = (double*)s->TimeDomain.d;
+  a.ActualState = (double*)s->ActualState.d;
+  a.ActualTime = (double*)s->ActualTime.d;
+  a.UnitParameters = (double*)s->UnitParameters.d;
+  a.SystemParameters = (double*)s->SystemParameters.d;
+  a.GlobalParameters = (double*)s->GlobalParameters.d;
+  a.IntegerGlobalParameters = (int*)s->IntegerGlobalParameters.d;
+  a.UnitAccessories = (double*)s->UnitAccessories.d;
+  a.IntegerUnitAccessories = (int*)s->IntegerUnitAccessories.d;
+  a.SystemAccessories = (double*)s->SystemAccessories.d;
+  a.IntegerSystemAccessories = (int*)s->IntegerSystemAccessories.d;
+  a.CouplingMatrix = s->d_CouplingFull;
+  a.DenseOutputIndex = (int*)s->DenseOutputIndex.d;
+  a.DenseOutputTimeInstances = (double*)s->DenseOutputTimeInstances.d;
+  a.DenseOutputStates = (double*)s->DenseOutputStates.d;
+  a.RelativeTolerance = s->d_tol;
+  a.AbsoluteTolerance = s->d_tol + ND;
+  a.EventTolerance = s->d_tol + 2 * ND;
+  a.EventDirection = s->d_dir;
+  a.Stats = s->d_stats;
+  a.InitialTimeStep = s->InitialTimeStep;
+  a.MaximumTimeStep = s->MaximumTimeStep;
+  a.MinimumTimeStep = s->MinimumTimeStep;
+  a.TimeStepGrowLimit = s->TimeStepGrowLimit;
+  a.TimeStepShrinkLimit = s->TimeStepShrinkLimit;
+  a.DenseOutputMinimumTimeStep = s->DenseOutputMinimumTimeStep;
+  a.DenseOutputSaveFrequency = s->DenseOutputSaveFrequency;
+  a.BiCGRhoTolerance = s->BiCGRhoTolerance;
+  a.BiCGResidualTolerance = s->BiCGResidualTolerance;
+  a.BiCGMaxIterations = s->BiCGMaxIterations;
+  a.ref_spb = s->d.SPB;
+  a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
+  int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
+  if (nsys <= 0) return BEX_OK;
+  dim3 grid((nsys + spb - 1) / spb), block(bexk::kBlock);
+  hipLaunchKernelGGL(fn, grid, block, lds, s->stream, a);
+  BEX_HIP(s, hipGetLastError());
+  return BEX_OK;
+}
+
+int bex_synchronise_device(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipDeviceSynchronize());
+  return BEX_OK;
+}
+
+int bex_insert_synchronisation_point(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipEventRecord(s->event, s->stream));
+  return BEX_OK;
+}
+
+int bex_synchronise_solver(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipEventSynchronize(s->event));
+  return BEX_OK;
+}
+
+int bex_get_stats(bex_solver* s, long long* out) {
+  if (!s || !out) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipMemcpyAsync(out, s->d_stats, sizeof(long long) * (size_t)s->d.NS * BEX_NSTAT,
+                            hipMemcpyDeviceToHost, s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  return BEX_OK;
+}
+
+int bex_reset_stats(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipMemsetAsync(s->d_stats, 0, sizeof(long long) * (size_t)s->d.NS * BEX_NSTAT, s->stream));
+  return BEX_OK;
+}
+
+}  // extern "C"

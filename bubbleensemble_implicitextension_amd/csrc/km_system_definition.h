@@ -1,0 +1,121 @@
+// Keller-Miksis bubble-ensemble system definition: the user hooks of the per-block solver.
+//
+// Same hook names, argument lists and semantics as the reference's system-definition header
+// (BubbleEnsemble_SystemDefinition.cuh:7-126). Differences:
+//   * the elementary functions come from bex_detmath.h (bit-identical on host and device)
+//     instead of the device math library;
+//   * the header also declares the model's dimensions (BexSystemDims), because the solver
+//     library is compiled for one system definition at a time and keeps unit parameters and
+//     accessories in registers and private memory sized at compile time.
+//
+// To solve a different model, write a header with the same hooks and BexSystemDims and build
+// the library with -DBEX_SYSTEM_DEFINITION='"your_header.h"'.
+#pragma once
+
+#include "bex_detmath.h"
+
+#ifndef PI
+#define PI 3.14159265358979323846
+#endif
+
+#ifndef NCOLL
+#define NCOLL 32  // recorded collapses per bubble (CollapseStrength.cu:22)
+#endif
+
+struct BexSystemDims {
+  static constexpr int UD = 2;            // unit dimension: x1 = R/R_E, x2 = dx1/dtau
+  static constexpr int NUP = 21;          // C0..C20 (CollapseStrength.cu:337-357)
+  static constexpr int NSP = 0;
+  static constexpr int NGP = 0;
+  static constexpr int NiGP = 0;
+  static constexpr int NUA = 2 * NCOLL;   // NCOLL maxima followed by NCOLL minima
+  static constexpr int NiUA = 1;          // event counter
+  static constexpr int NSA = 0;
+  static constexpr int NiSA = 0;
+  static constexpr int NE = 1;            // dR/dtau = 0
+  static constexpr int NC = 1;
+};
+
+// SYSTEM (BubbleEnsemble_SystemDefinition.cuh:7-49)
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_OdeFunction(
+    int sid, int uid, Precision* F, Precision* X, Precision T, Precision* uPAR,
+    Precision* sPAR, Precision* gPAR, int* igPAR, Precision* uACC, int* iuACC,
+    Precision* sACC, int* isACC, Precision* CPT, Precision* CPF) {
+  Precision rx1 = 1.0 / X[0];
+  Precision p = bex::pow(rx1, uPAR[10]);
+
+  Precision s1;
+  Precision c1;
+  bex::sincospi(2.0 * T, &s1, &c1);
+
+  Precision s2 = bex::sin(2.0 * uPAR[11] * PI * T + uPAR[12]);
+  Precision c2 = bex::cos(2.0 * uPAR[11] * PI * T + uPAR[12]);
+
+  Precision N = (uPAR[0] + uPAR[1] * X[1]) * p - uPAR[2] * (1.0 + uPAR[9] * X[1]) -
+                uPAR[3] * rx1 - uPAR[4] * X[1] * rx1 -
+                1.5 * (1.0 - uPAR[9] * X[1] / 3.0) * X[1] * X[1] -
+                (uPAR[5] * s1 + uPAR[6] * s2) * (1.0 + uPAR[9] * X[1]) -
+                X[0] * (uPAR[7] * c1 + uPAR[8] * c2);
+  Precision D = X[0] - uPAR[9] * X[0] * X[1] + uPAR[4] * uPAR[9];
+  Precision rD = 1.0 / D;
+
+  // Uncoupled part; F[1] is the unknown of the implicit coupling solve.
+  F[0] = X[1];
+  F[1] = N * rD;
+
+  // Explicit part of the coupling.
+  CPT[0] = -2 * X[0] * X[1] * X[1];
+
+  // Row factor of the implicit coupling matrix.
+  CPF[0] = rD;
+}
+
+// EVENTS (BubbleEnsemble_SystemDefinition.cuh:68-76): local extrema of R.
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_EventFunction(
+    int sid, int uid, Precision* EF, Precision T, Precision dT, Precision* TD, Precision* X,
+    Precision* uPAR, Precision* sPAR, Precision* gPAR, int* igPAR, Precision* uACC,
+    int* iuACC, Precision* sACC, int* isACC) {
+  EF[0] = X[1];
+}
+
+// Called after ActionAfterSuccessfulTimeStep (BubbleEnsemble_SystemDefinition.cuh:78-92):
+// store the maximum and start tracking the following minimum.
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_ActionAfterEventDetection(
+    int sid, int uid, int IDX, int& UDT, Precision& T, Precision& dT, Precision* TD,
+    Precision* X, Precision* uPAR, Precision* sPAR, Precision* gPAR, int* igPAR,
+    Precision* uACC, int* iuACC, Precision* sACC, int* isACC) {
+  iuACC[0]++;
+
+  if (iuACC[0] < NCOLL) {
+    uACC[iuACC[0]] = X[0];
+    uACC[iuACC[0] + NCOLL] = X[0];
+  }
+}
+
+// ACCESSORIES (BubbleEnsemble_SystemDefinition.cuh:96-105): running minimum after a maximum.
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_ActionAfterSuccessfulTimeStep(
+    int sid, int uid, int& UDT, Precision& T, Precision& dT, Precision* TD, Precision* X,
+    Precision* uPAR, Precision* sPAR, Precision* gPAR, int* igPAR, Precision* uACC,
+    int* iuACC, Precision* sACC, int* isACC) {
+  if ((iuACC[0] < NCOLL) && (X[0] < uACC[iuACC[0] + NCOLL])) uACC[iuACC[0] + NCOLL] = X[0];
+}
+
+// BubbleEnsemble_SystemDefinition.cuh:107-116: every Solve() restarts at TD[0].
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_Initialization(
+    int sid, int uid, int& DOIDX, Precision& T, Precision& dT, Precision* TD, Precision* X,
+    Precision* uPAR, Precision* sPAR, Precision* gPAR, int* igPAR, Precision* uACC,
+    int* iuACC, Precision* sACC, int* isACC) {
+  if (uid == 0) T = TD[0];
+}
+
+// BubbleEnsemble_SystemDefinition.cuh:118-126
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_Finalization(
+    int sid, int uid, int& DOIDX, Precision& T, Precision& dT, Precision* TD, Precision* X,
+    Precision* uPAR, Precision* sPAR, Precision* gPAR, int* igPAR, Precision* uACC,
+    int* iuACC, Precision* sACC, int* isACC) {}

@@ -1,0 +1,224 @@
+"""Python mirror of the reference's ProblemSolver host API over libbex.so.
+
+Same enumerations, member names, overloads and argument meaning as
+SourceCodes/CoupledSystems_PerBlock_Interface.cuh:33-248. Errors the reference treats as fatal
+(bound checks, wrong overload/variable combinations, runtime failures) raise ``BexError``
+instead of calling exit(); the C++ header CoupledSystems_PerBlock_Interface.hpp restores the
+print-and-exit behaviour for C++ callers.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+
+import numpy as np
+
+from . import _native
+
+
+class Algorithms(enum.IntEnum):  # Interface.cuh:33
+    RK4 = 0
+    RKCK45 = 1
+
+
+class ListOfVariables(enum.IntEnum):  # Interface.cuh:35-53
+    All = 0
+    TimeDomain = 1
+    ActualTime = 2
+    ActualState = 3
+    UnitParameters = 4
+    SystemParameters = 5
+    GlobalParameters = 6
+    IntegerGlobalParameters = 7
+    UnitAccessories = 8
+    IntegerUnitAccessories = 9
+    SystemAccessories = 10
+    IntegerSystemAccessories = 11
+    CouplingMatrix = 12
+    CouplingStrength = 13
+    CouplingIndex = 14
+    DenseOutput = 15
+    DenseIndex = 16
+    DenseTime = 17
+    DenseState = 18
+
+
+class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
+    InitialTimeStep = 0
+    ActiveSystems = 1
+    MaximumTimeStep = 2
+    MinimumTimeStep = 3
+    TimeStepGrowLimit = 4
+    TimeStepShrinkLimit = 5
+    RelativeTolerance = 6
+    AbsoluteTolerance = 7
+    EventTolerance = 8
+    EventDirection = 9
+    DenseOutputSaveFrequency = 10
+    DenseOutputMinimumTimeStep = 11
+    SharedGlobalVariables = 12
+    SharedCouplingMatrices = 13
+    BiCGRhoTolerance = 14
+    BiCGResidualTolerance = 15
+    BiCGMaxIterations = 16
+
+
+# Reference-style bare names (the reference uses unscoped enums).
+globals().update({m.name: m for m in Algorithms})
+globals().update({m.name: m for m in ListOfVariables})
+globals().update({m.name: m for m in ListOfSolverOptions})
+
+_UNIT_VARS = {ListOfVariables.ActualState, ListOfVariables.UnitParameters,
+              ListOfVariables.UnitAccessories, ListOfVariables.IntegerUnitAccessories}
+
+
+def compiled_dims():
+    d = _native.ProblemDesc()
+    _native.check(_native.load().bex_compiled_dims(ctypes.byref(d)))
+    return {f: getattr(d, f) for f, _ in d._fields_}
+
+
+class ProblemSolver:
+    """One solver object: pinned host + device buffers for NS systems of UPS units.
+
+    Constructor arguments are the reference's template parameters
+    (Interface.cuh:135) plus the device ordinal.
+    """
+
+    def __init__(self, NS, UPS, UD=2, TPB=None, SPB=1, NC=1, CBW=0, CCI=0, NUP=21, NSP=0, NGP=0,
+                 NiGP=0, NUA=64, NiUA=1, NSA=0, NiSA=0, NE=1, NDO=0,
+                 Algorithm=Algorithms.RKCK45, device=0):
+        if TPB is None:
+            TPB = SPB * UPS
+        self.desc = _native.ProblemDesc(NS, UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP, NiGP,
+                                        NUA, NiUA, NSA, NiSA, NE, NDO, int(Algorithm))
+        self._lib = _native.load()
+        h = ctypes.c_void_p()
+        _native.check(self._lib.bex_create(ctypes.byref(self.desc), device, ctypes.byref(h)))
+        self._h = h
+        self.NS, self.UPS, self.UD, self.NDO = NS, UPS, UD, NDO
+        self.dims = {f: getattr(self.desc, f) for f, _ in self.desc._fields_}
+
+    # -- lifetime --------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bex_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        return _native.check(rc, self._h)
+
+    # -- options (Interface.cuh:771-865) ------------------------------------------------
+    def SolverOption(self, option, *args):
+        option = ListOfSolverOptions(option)
+        if len(args) == 1:
+            self._check(self._lib.bex_solver_option(self._h, int(option), float(args[0])))
+        elif len(args) == 2:
+            self._check(self._lib.bex_solver_option_indexed(self._h, int(option), int(args[0]),
+                                                             float(args[1])))
+        else:
+            raise TypeError("SolverOption(option, value) or SolverOption(option, index, value)")
+
+    # -- element access (Interface.cuh:867-1140, 1338-1529) -------------------------------
+    def SetHost(self, *args):
+        fn, call = self._dispatch(args, setter=True)
+        self._check(fn(self._h, *call))
+
+    def GetHost(self, *args):
+        out = ctypes.c_double()
+        fn, call = self._dispatch(args + (None,), setter=False)
+        self._check(fn(self._h, *call[:-1], ctypes.byref(out)))
+        return out.value
+
+    def _dispatch(self, args, setter):
+        L = self._lib
+        pre = "bex_set_host_" if setter else "bex_get_host_"
+        isvar = [isinstance(a, ListOfVariables) for a in args]
+        n = len(args)
+        v = args[-1]
+        val = float(v) if setter else None
+        if n == 6 and isvar[2]:  # (sid, uid, DenseState, comp, k, v)
+            return getattr(L, pre + "dense_state"), [int(args[0]), int(args[1]), int(args[2]),
+                                                     int(args[3]), int(args[4]), val]
+        if n == 5 and isvar[2]:  # (sid, uid, var, k, v)
+            return getattr(L, pre + "unit"), [int(args[0]), int(args[1]), int(args[2]),
+                                              int(args[3]), val]
+        if n == 5 and isvar[1]:  # (c, CouplingMatrix, row, col, v)
+            return getattr(L, pre + "coupling"), [int(args[0]), int(args[1]), int(args[2]),
+                                                  int(args[3]), val]
+        if n == 4 and isvar[1]:  # (sid, var, k, v)
+            return getattr(L, pre + "system"), [int(args[0]), int(args[1]), int(args[2]), val]
+        if n == 3 and isvar[0]:  # (var, k, v)
+            return getattr(L, pre + "global"), [int(args[0]), int(args[1]), val]
+        if n == 3 and isvar[1]:  # (sid, DenseIndex, v)
+            return getattr(L, pre + "dense_index"), [int(args[0]), int(args[1]), val]
+        raise TypeError("no SetHost/GetHost overload takes these arguments")
+
+    def host_view(self, var):
+        """Zero-copy numpy view of a pinned host buffer in the reference layout."""
+        ptr = ctypes.c_void_p()
+        count = ctypes.c_longlong()
+        esz = ctypes.c_int()
+        self._check(self._lib.bex_host_view(self._h, int(ListOfVariables(var)), ctypes.byref(ptr),
+                                            ctypes.byref(count), ctypes.byref(esz)))
+        if count.value == 0:
+            return np.zeros(0, dtype=np.float64 if esz.value == 8 else np.int32)
+        ctype = ctypes.c_double if esz.value == 8 else ctypes.c_int
+        arr = (ctype * count.value).from_address(ptr.value)
+        view = np.ctypeslib.as_array(arr)
+        view.flags.writeable = True
+        return view
+
+    # -- synchronisation (Interface.cuh:1142-1336, 1949-1983) -----------------------------
+    def SynchroniseFromHostToDevice(self, var):
+        self._check(self._lib.bex_sync_h2d(self._h, int(ListOfVariables(var))))
+
+    def SynchroniseFromDeviceToHost(self, var):
+        self._check(self._lib.bex_sync_d2h(self._h, int(ListOfVariables(var))))
+
+    def Solve(self):
+        self._check(self._lib.bex_solve(self._h))
+
+    def SynchroniseDevice(self):
+        self._check(self._lib.bex_synchronise_device(self._h))
+
+    def InsertSynchronisationPoint(self):
+        self._check(self._lib.bex_insert_synchronisation_point(self._h))
+
+    def SynchroniseSolver(self):
+        self._check(self._lib.bex_synchronise_solver(self._h))
+
+    # -- counters ----------------------------------------------------------------------
+    def stats(self):
+        out = np.zeros(self.NS * _native.NSTAT, dtype=np.int64)
+        self._check(self._lib.bex_get_stats(self._h,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        return out.reshape(self.NS, _native.NSTAT)
+
+    def reset_stats(self):
+        self._check(self._lib.bex_reset_stats(self._h))
+
+    # -- convenience views (reference layout with TPB == SPB*UPS) --------------------------
+    def state(self):
+        return self.host_view(ListOfVariables.ActualState).reshape(
+            self.UD, self.NS, self.UPS).transpose(1, 2, 0)
+
+    def accessories(self):
+        v = self.host_view(ListOfVariables.UnitAccessories)
+        return v.reshape(-1, self.NS, self.UPS).transpose(1, 2, 0)
+
+    def event_counts(self):
+        v = self.host_view(ListOfVariables.IntegerUnitAccessories)
+        return v.reshape(-1, self.NS, self.UPS)[0] + 1

@@ -1,0 +1,153 @@
+/* C ABI of the MI355X coupled Keller-Miksis ensemble solver (libbex.so).
+ *
+ * One opaque handle replaces one reference `ProblemSolver<...>` object
+ * (SourceCodes/CoupledSystems_PerBlock_Interface.cuh:135-248). The handle owns pinned host
+ * and device buffers in the reference's structure-of-arrays layout; values are exchanged by
+ * copy through the Set/Get calls or in bulk through host views, and moved between host and
+ * device explicitly, in stream order, exactly as in the reference.
+ *
+ * Every call returns BEX_OK (0) or a negative BEX_E_* code; bex_last_error() returns the
+ * message. The C++ header CoupledSystems_PerBlock_Interface.hpp turns these codes back into
+ * the reference's print-and-exit behaviour.
+ *
+ * Enumerations keep the reference's numeric order:
+ *   Algorithms          Interface.cuh:33
+ *   ListOfVariables     Interface.cuh:35-53
+ *   ListOfSolverOptions Interface.cuh:55-68 (extended after SharedCouplingMatrices)
+ */
+#ifndef BEX_CAPI_H
+#define BEX_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEX_OK 0
+#define BEX_E_ARGUMENT -1    /* null handle/pointer, bad enum value */
+#define BEX_E_BOUND -2       /* index outside its range (Interface.cuh:718-733 BoundCheck) */
+#define BEX_E_VARIABLE -3    /* variable not valid for this overload (Interface.cuh:903-908) */
+#define BEX_E_OPTION -4      /* option not valid for this overload (Interface.cuh:811-817) */
+#define BEX_E_MEMORY -5      /* not enough device memory (Interface.cuh:463-471) */
+#define BEX_E_HIP -6         /* HIP runtime error (Interface.cuh:13-22 gpuErrCHK) */
+#define BEX_E_UNSUPPORTED -7 /* shape or dimension this build does not support */
+#define BEX_E_MISMATCH -8    /* dimensions differ from the compiled system definition */
+
+/* Algorithms (Interface.cuh:33) */
+#define BEX_RK4 0
+#define BEX_RKCK45 1
+
+/* ListOfVariables (Interface.cuh:35-53) */
+enum bex_variable {
+  BEX_All = 0,
+  BEX_TimeDomain,
+  BEX_ActualTime,
+  BEX_ActualState,
+  BEX_UnitParameters,
+  BEX_SystemParameters,
+  BEX_GlobalParameters,
+  BEX_IntegerGlobalParameters,
+  BEX_UnitAccessories,
+  BEX_IntegerUnitAccessories,
+  BEX_SystemAccessories,
+  BEX_IntegerSystemAccessories,
+  BEX_CouplingMatrix,
+  BEX_CouplingStrength,
+  BEX_CouplingIndex,
+  BEX_DenseOutput,
+  BEX_DenseIndex,
+  BEX_DenseTime,
+  BEX_DenseState
+};
+
+/* ListOfSolverOptions (Interface.cuh:55-68), then this library's additions. */
+enum bex_option {
+  BEX_InitialTimeStep = 0,
+  BEX_ActiveSystems,
+  BEX_MaximumTimeStep,
+  BEX_MinimumTimeStep,
+  BEX_TimeStepGrowLimit,
+  BEX_TimeStepShrinkLimit,
+  BEX_RelativeTolerance,
+  BEX_AbsoluteTolerance,
+  BEX_EventTolerance,
+  BEX_EventDirection,
+  BEX_DenseOutputSaveFrequency,
+  BEX_DenseOutputMinimumTimeStep,
+  BEX_SharedGlobalVariables,
+  BEX_SharedCouplingMatrices,
+  /* additions: the reference hard-codes these BiCG constants (Steppers.cuh:508,522,558) */
+  BEX_BiCGRhoTolerance,
+  BEX_BiCGResidualTolerance,
+  BEX_BiCGMaxIterations
+};
+
+/* The reference's template parameters (Interface.cuh:135). */
+typedef struct {
+  int NS, UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP, NiGP, NUA, NiUA, NSA, NiSA, NE, NDO;
+  int Algorithm;
+} bex_problem_desc;
+
+typedef struct bex_solver bex_solver;
+
+/* Dimensions of the system definition compiled into this library (UD, NUP, NSP, NGP, NiGP,
+ * NUA, NiUA, NSA, NiSA, NE, NC); the remaining fields are zero. No reference counterpart:
+ * the reference compiles the system definition into the application instead. */
+int bex_compiled_dims(bex_problem_desc* out);
+
+/* ProblemSolver::ProblemSolver(int) (Interface.cuh:352-657) */
+int bex_create(const bex_problem_desc* desc, int device, bex_solver** out);
+/* ProblemSolver::~ProblemSolver() (Interface.cuh:660-714) */
+int bex_destroy(bex_solver* s);
+
+/* ProblemSolver::SolverOption(ListOfSolverOptions, T) (Interface.cuh:771-818) */
+int bex_solver_option(bex_solver* s, int option, double value);
+/* ProblemSolver::SolverOption(ListOfSolverOptions, int, T) (Interface.cuh:820-865) */
+int bex_solver_option_indexed(bex_solver* s, int option, int index, double value);
+
+/* SetHost overloads (Interface.cuh:867-1140); integer variables take the value cast to int. */
+int bex_set_host_unit(bex_solver* s, int sid, int uid, int var, int k, double v);     /* :867-912 */
+int bex_set_host_system(bex_solver* s, int sid, int var, int k, double v);            /* :914-960 */
+int bex_set_host_global(bex_solver* s, int var, int k, double v);                     /* :962-996 */
+int bex_set_host_coupling(bex_solver* s, int c, int var, int row, int col, double v); /* :998-1086 */
+int bex_set_host_dense_index(bex_solver* s, int sid, int var, double v);              /* :1088-1107 */
+int bex_set_host_dense_state(bex_solver* s, int sid, int uid, int var, int comp, int k,
+                             double v);                                                /* :1109-1140 */
+
+/* GetHost overloads (Interface.cuh:1338-1529) */
+int bex_get_host_unit(bex_solver* s, int sid, int uid, int var, int k, double* v);
+int bex_get_host_system(bex_solver* s, int sid, int var, int k, double* v);
+int bex_get_host_global(bex_solver* s, int var, int k, double* v);
+int bex_get_host_coupling(bex_solver* s, int c, int var, int row, int col, double* v);
+int bex_get_host_dense_index(bex_solver* s, int sid, int var, double* v);
+int bex_get_host_dense_state(bex_solver* s, int sid, int uid, int var, int comp, int k,
+                             double* v);
+
+/* Bulk access to a host buffer in the reference layout (no reference counterpart; the
+ * reference only exposes element-wise Set/Get). *elem_bytes is 8 (double) or 4 (int). */
+int bex_host_view(bex_solver* s, int var, void** ptr, long long* count, int* elem_bytes);
+
+/* ProblemSolver::SynchroniseFromHostToDevice / FromDeviceToHost (Interface.cuh:1142-1336) */
+int bex_sync_h2d(bex_solver* s, int var);
+int bex_sync_d2h(bex_solver* s, int var);
+
+/* ProblemSolver::Solve() (Interface.cuh:1949-1956): one asynchronous launch. */
+int bex_solve(bex_solver* s);
+/* ProblemSolver::SynchroniseDevice / InsertSynchronisationPoint / SynchroniseSolver
+ * (Interface.cuh:1958-1983) */
+int bex_synchronise_device(bex_solver* s);
+int bex_insert_synchronisation_point(bex_solver* s);
+int bex_synchronise_solver(bex_solver* s);
+
+/* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);
+ * copies device counters to `out` (synchronous). No reference counterpart. */
+int bex_get_stats(bex_solver* s, long long* out);
+int bex_reset_stats(bex_solver* s);
+
+/* Message of the last failing call on this handle (or of the last bex_create failure when
+ * s is NULL). */
+const char* bex_last_error(const bex_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
