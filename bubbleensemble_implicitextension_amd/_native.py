@@ -65,6 +65,8 @@ _SIGNATURES = {
     "bex_synchronise_solver": ([ctypes.c_void_p], ctypes.c_int),
     "bex_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "bex_reset_stats": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_timer_start": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_timer_stop": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
 }
 

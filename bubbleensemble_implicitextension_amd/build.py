@@ -50,5 +50,25 @@ def build_library(verbose=False, extra_flags=(), output=LIB):
     return output
 
 
+def build_apps(verbose=False):
+    """C++ applications written against include/CoupledSystems_PerBlock_Interface.hpp."""
+    app = os.path.join(HERE, "collapse_strength")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(REPO, "include"),
+           os.path.join(HERE, "apps", "collapse_strength.cpp"), "-L" + HERE, "-lbex",
+           "-Wl,-rpath,$ORIGIN", "-o", app]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if verbose or res.returncode != 0:
+        sys.stdout.write(res.stdout)
+        sys.stderr.write(res.stderr)
+    if res.returncode != 0:
+        raise RuntimeError("building collapse_strength failed")
+    return app
+
+
+def build_all(verbose=False):
+    build_library(verbose=verbose)
+    build_apps(verbose=verbose)
+
+
 if __name__ == "__main__":
-    build_library(verbose="--verbose" in sys.argv)
+    build_all(verbose="--verbose" in sys.argv)

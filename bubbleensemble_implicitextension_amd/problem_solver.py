@@ -207,6 +207,14 @@ class ProblemSolver:
                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
         return out.reshape(self.NS, _native.NSTAT)
 
+    def timer_start(self):
+        self._check(self._lib.bex_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        self._check(self._lib.bex_timer_stop(self._h, ctypes.byref(ms)))
+        return ms.value
+
     def reset_stats(self):
         self._check(self._lib.bex_reset_stats(self._h))
 

@@ -1,0 +1,265 @@
+// Drop-in C++ host interface of the MI355X coupled-systems per-block solver.
+//
+// Same class template, enumerations, member functions, overloads, argument meaning and error
+// behaviour as the reference's host interface
+// (SourceCodes/CoupledSystems_PerBlock_Interface.cuh:33-248): an application written against
+// the reference's ProblemSolver compiles against this header unchanged (with Precision =
+// double) and links against libbex.so, whose kernel is built for the system definition the
+// application would otherwise #include before the interface.
+//
+// Errors the reference treats as fatal (bound checks, wrong overload/variable combination,
+// HIP failures, memory shortfall) print a message to std::cerr and exit(EXIT_FAILURE), as in
+// Interface.cuh:13-22 and :718-733.
+#ifndef COUPLEDSYSTEMS_PERBLOCK_INTERFACE_HPP
+#define COUPLEDSYSTEMS_PERBLOCK_INTERFACE_HPP
+
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <type_traits>
+
+#include "bex_capi.h"
+
+enum Algorithms { RK4 = BEX_RK4, RKCK45 = BEX_RKCK45 };  // Interface.cuh:33
+
+enum ListOfVariables {  // Interface.cuh:35-53
+  All = BEX_All,
+  TimeDomain,
+  ActualTime,
+  ActualState,
+  UnitParameters,
+  SystemParameters,
+  GlobalParameters,
+  IntegerGlobalParameters,
+  UnitAccessories,
+  IntegerUnitAccessories,
+  SystemAccessories,
+  IntegerSystemAccessories,
+  CouplingMatrix,
+  CouplingStrength,
+  CouplingIndex,
+  DenseOutput,
+  DenseIndex,
+  DenseTime,
+  DenseState
+};
+
+enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
+  InitialTimeStep = BEX_InitialTimeStep,
+  ActiveSystems,
+  MaximumTimeStep,
+  MinimumTimeStep,
+  TimeStepGrowLimit,
+  TimeStepShrinkLimit,
+  RelativeTolerance,
+  AbsoluteTolerance,
+  EventTolerance,
+  EventDirection,
+  DenseOutputSaveFrequency,
+  DenseOutputMinimumTimeStep,
+  SharedGlobalVariables,
+  SharedCouplingMatrices,
+  BiCGRhoTolerance,
+  BiCGResidualTolerance,
+  BiCGMaxIterations
+};
+
+template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,
+          int NGP, int NiGP, int NUA, int NiUA, int NSA, int NiSA, int NE, int NDO,
+          Algorithms Algorithm, class Precision>
+class ProblemSolver {
+  static_assert(std::is_same<Precision, double>::value,
+                "the MI355X solver integrates in FP64 (Precision = double)");
+
+ private:
+  bex_solver* handle_ = nullptr;
+
+  void Check(int rc, const char* function) const {
+    if (rc == BEX_OK) return;
+    std::cerr << "ERROR: In solver member function " << function << "!" << std::endl;
+    std::cerr << "       " << bex_last_error(handle_) << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+
+  void WriteToFile(const std::string& name, ListOfVariables var, int columns) {
+    void* ptr = nullptr;
+    long long count = 0;
+    int esz = 8;
+    Check(bex_host_view(handle_, var, &ptr, &count, &esz), "Print");
+    std::ofstream f(name.c_str());
+    f.precision(14);
+    f.flags(std::ios::scientific);
+    const long long rows = columns > 0 ? count / columns : 0;
+    for (long long r = 0; r < rows; r++) {
+      for (int c = 0; c < columns; c++) {
+        long long i = r + (long long)c * rows;
+        if (esz == 8)
+          f << std::setw(23) << static_cast<double*>(ptr)[i] << ' ';
+        else
+          f << std::setw(8) << static_cast<int*>(ptr)[i] << ' ';
+      }
+      f << '\n';
+    }
+  }
+
+ public:
+  // Interface.cuh:352-657
+  explicit ProblemSolver(int device) {
+    bex_problem_desc d{NS,  UPS, UD,   TPB, SPB,  NC,  CBW, CCI, NUP,      NSP,
+                       NGP, NiGP, NUA, NiUA, NSA, NiSA, NE,  NDO, Algorithm};
+    int rc = bex_create(&d, device, &handle_);
+    if (rc != BEX_OK) {
+      std::cerr << "ERROR: creating the solver object failed!" << std::endl;
+      std::cerr << "       " << bex_last_error(nullptr) << std::endl;
+      std::exit(EXIT_FAILURE);
+    }
+  }
+  // Interface.cuh:660-714
+  ~ProblemSolver() {
+    if (handle_) bex_destroy(handle_);
+  }
+  ProblemSolver(const ProblemSolver&) = delete;
+  ProblemSolver& operator=(const ProblemSolver&) = delete;
+
+  // Interface.cuh:771-865
+  template <typename T>
+  void SolverOption(ListOfSolverOptions option, T value) {
+    Check(bex_solver_option(handle_, option, (double)value), "SolverOption");
+  }
+  template <typename T>
+  void SolverOption(ListOfSolverOptions option, int index, T value) {
+    Check(bex_solver_option_indexed(handle_, option, index, (double)value), "SolverOption");
+  }
+
+  // Interface.cuh:867-1140
+  template <typename T>
+  void SetHost(int sid, int uid, ListOfVariables var, int k, T v) {
+    Check(bex_set_host_unit(handle_, sid, uid, var, k, (double)v), "SetHost");
+  }
+  template <typename T>
+  void SetHost(int sid, ListOfVariables var, int k, T v) {
+    Check(bex_set_host_system(handle_, sid, var, k, (double)v), "SetHost");
+  }
+  template <typename T>
+  void SetHost(ListOfVariables var, int k, T v) {
+    Check(bex_set_host_global(handle_, var, k, (double)v), "SetHost");
+  }
+  template <typename T>
+  void SetHost(int c, ListOfVariables var, int row, int col, T v) {
+    Check(bex_set_host_coupling(handle_, c, var, row, col, (double)v), "SetHost");
+  }
+  template <typename T>
+  void SetHost(int sid, ListOfVariables var, T v) {
+    Check(bex_set_host_dense_index(handle_, sid, var, (double)v), "SetHost");
+  }
+  template <typename T>
+  void SetHost(int sid, int uid, ListOfVariables var, int comp, int k, T v) {
+    Check(bex_set_host_dense_state(handle_, sid, uid, var, comp, k, (double)v), "SetHost");
+  }
+
+  // Interface.cuh:1142-1336
+  void SynchroniseFromHostToDevice(ListOfVariables var) {
+    Check(bex_sync_h2d(handle_, var), "SynchroniseFromHostToDevice");
+  }
+  void SynchroniseFromDeviceToHost(ListOfVariables var) {
+    Check(bex_sync_d2h(handle_, var), "SynchroniseFromDeviceToHost");
+  }
+
+  // Interface.cuh:1338-1529
+  template <typename T>
+  T GetHost(int sid, int uid, ListOfVariables var, int k) {
+    double v;
+    Check(bex_get_host_unit(handle_, sid, uid, var, k, &v), "GetHost");
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, ListOfVariables var, int k) {
+    double v;
+    Check(bex_get_host_system(handle_, sid, var, k, &v), "GetHost");
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(ListOfVariables var, int k) {
+    double v;
+    Check(bex_get_host_global(handle_, var, k, &v), "GetHost");
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int c, ListOfVariables var, int row, int col) {
+    double v;
+    Check(bex_get_host_coupling(handle_, c, var, row, col, &v), "GetHost");
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, ListOfVariables var) {
+    double v;
+    Check(bex_get_host_dense_index(handle_, sid, var, &v), "GetHost");
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, int uid, ListOfVariables var, int comp, int k) {
+    double v;
+    Check(bex_get_host_dense_state(handle_, sid, uid, var, comp, k, &v), "GetHost");
+    return (T)v;
+  }
+
+  // Interface.cuh:1531-1947: host buffers to text files, one row per logical thread (unit
+  // scope) or per system (system scope), one column per component.
+  void Print(ListOfVariables var) {
+    int cols = 1;
+    switch (var) {
+      case ActualState: cols = UD; break;
+      case UnitParameters: cols = NUP; break;
+      case UnitAccessories: cols = NUA; break;
+      case IntegerUnitAccessories: cols = NiUA; break;
+      case TimeDomain: cols = 2; break;
+      case SystemParameters: cols = NSP; break;
+      case SystemAccessories: cols = NSA; break;
+      case IntegerSystemAccessories: cols = NiSA; break;
+      case CouplingStrength: cols = NC; break;
+      default: cols = 1; break;
+    }
+    WriteToFile(std::string(VariableName(var)) + ".txt", var, cols);
+  }
+  void Print(ListOfVariables var, int serial) {
+    WriteToFile(std::string(VariableName(var)) + "_" + std::to_string(serial) + ".txt", var, 1);
+  }
+
+  // Interface.cuh:1949-1983
+  void Solve() { Check(bex_solve(handle_), "Solve"); }
+  void SynchroniseDevice() { Check(bex_synchronise_device(handle_), "SynchroniseDevice"); }
+  void InsertSynchronisationPoint() {
+    Check(bex_insert_synchronisation_point(handle_), "InsertSynchronisationPoint");
+  }
+  void SynchroniseSolver() { Check(bex_synchronise_solver(handle_), "SynchroniseSolver"); }
+
+  // Additions: the underlying handle and the per-system counters.
+  bex_solver* Handle() { return handle_; }
+
+  static const char* VariableName(ListOfVariables v) {
+    static const char* names[] = {"All",
+                                  "TimeDomain",
+                                  "ActualTime",
+                                  "ActualState",
+                                  "UnitParameters",
+                                  "SystemParameters",
+                                  "GlobalParameters",
+                                  "IntegerGlobalParameters",
+                                  "UnitAccessories",
+                                  "IntegerUnitAccessories",
+                                  "SystemAccessories",
+                                  "IntegerSystemAccessories",
+                                  "CouplingMatrix",
+                                  "CouplingStrength",
+                                  "CouplingIndex",
+                                  "DenseOutput",
+                                  "DenseIndex",
+                                  "DenseTime",
+                                  "DenseState"};
+    return names[(int)v];
+  }
+};
+
+#endif

@@ -143,6 +143,12 @@ int bex_synchronise_solver(bex_solver* s);
 int bex_get_stats(bex_solver* s, long long* out);
 int bex_reset_stats(bex_solver* s);
 
+/* Kernel timing on the solver's own stream: bex_timer_start records a HIP event, bex_timer_stop
+ * records a second one, waits for it and returns the elapsed milliseconds. No reference
+ * counterpart (the reference times its drivers with clock()). */
+int bex_timer_start(bex_solver* s);
+int bex_timer_stop(bex_solver* s, double* elapsed_ms);
+
 /* Message of the last failing call on this handle (or of the last bex_create failure when
  * s is NULL). */
 const char* bex_last_error(const bex_solver* s);

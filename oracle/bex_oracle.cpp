@@ -260,6 +260,115 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
 }
 
 // ---------------------------------------------------------------------------------------
+// Cash-Karp RKCK45 step (Steppers.cuh:153-424) for n values with stage(s, Ts) filling K[s].
+// Returns 0 if the new state or the error estimate is not finite (Steppers.cuh:420-421).
+template <class Stage>
+int rkck45_step(int n, const double* X, double t, double dt, double* K, double* Xs, double* Xn,
+                double* Err, Stage stage) {
+  double* k1 = &K[0];
+  double* k2 = &K[n];
+  double* k3 = &K[2 * n];
+  double* k4 = &K[3 * n];
+  double* k5 = &K[4 * n];
+  double* k6 = &K[5 * n];
+  int isFinite = 1;
+  for (int i = 0; i < n; i++) Xs[i] = X[i];
+  stage(0, t);
+  for (int i = 0; i < n; i++) Xs[i] = X[i] + (k1[i] * (1.0 / 5.0)) * dt;
+  stage(1, t + (1.0 / 5.0) * dt);
+  for (int i = 0; i < n; i++) Xs[i] = X[i] + (k1[i] * (3.0 / 40.0) + k2[i] * (9.0 / 40.0)) * dt;
+  stage(2, t + (3.0 / 10.0) * dt);
+  for (int i = 0; i < n; i++)
+    Xs[i] = X[i] + (k1[i] * (3.0 / 10.0) + k2[i] * (-9.0 / 10.0) + k3[i] * (6.0 / 5.0)) * dt;
+  stage(3, t + (3.0 / 5.0) * dt);
+  for (int i = 0; i < n; i++)
+    Xs[i] = X[i] + (k1[i] * (-11.0 / 54.0) + k2[i] * (5.0 / 2.0) + k3[i] * (-70.0 / 27.0) +
+                    k4[i] * (35.0 / 27.0)) *
+                       dt;
+  stage(4, t + dt);
+  for (int i = 0; i < n; i++)
+    Xs[i] = X[i] + (k1[i] * (1631.0 / 55296.0) + k2[i] * (175.0 / 512.0) +
+                    k3[i] * (575.0 / 13824.0) + k4[i] * (44275.0 / 110592.0) +
+                    k5[i] * (253.0 / 4096.0)) *
+                       dt;
+  stage(5, t + (7.0 / 8.0) * dt);
+  for (int i = 0; i < n; i++) {
+    Xn[i] = X[i] + (k1[i] * (37.0 / 378.0) + k3[i] * (250.0 / 621.0) + k4[i] * (125.0 / 594.0) +
+                    k6[i] * (512.0 / 1771.0)) *
+                       dt;
+    double e = k1[i] * (37.0 / 378.0 - 2825.0 / 27648.0) +
+               k3[i] * (250.0 / 621.0 - 18575.0 / 48384.0) +
+               k4[i] * (125.0 / 594.0 - 13525.0 / 55296.0) + k5[i] * (0.0 - 277.0 / 14336.0) +
+               k6[i] * (512.0 / 1771.0 - 1.0 / 4.0);
+    Err[i] = dt * std::fabs(e) + 1e-18;
+    if (!std::isfinite(Xn[i]) || !std::isfinite(Err[i])) isFinite = 0;
+  }
+  return isFinite;
+}
+
+// Classical RK4 step. The reference declares RK4 (Interface.cuh:33) and its controller
+// (ErrorControllers.cuh:424-450) but has no per-block RK4 stepper.
+template <class Stage>
+int rk4_step(int n, const double* X, double t, double dt, double* K, double* Xs, double* Xn,
+             Stage stage) {
+  double* k1 = &K[0];
+  double* k2 = &K[n];
+  double* k3 = &K[2 * n];
+  double* k4 = &K[3 * n];
+  int isFinite = 1;
+  for (int i = 0; i < n; i++) Xs[i] = X[i];
+  stage(0, t);
+  for (int i = 0; i < n; i++) Xs[i] = X[i] + k1[i] * (0.5 * dt);
+  stage(1, t + 0.5 * dt);
+  for (int i = 0; i < n; i++) Xs[i] = X[i] + k2[i] * (0.5 * dt);
+  stage(2, t + 0.5 * dt);
+  for (int i = 0; i < n; i++) Xs[i] = X[i] + k3[i] * dt;
+  stage(3, t + dt);
+  for (int i = 0; i < n; i++) {
+    Xn[i] = X[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) * (dt * (1.0 / 6.0));
+    if (!std::isfinite(Xn[i])) isFinite = 0;
+  }
+  return isFinite;
+}
+
+// Step-size control of the RKCK45 path after the reductions
+// (ErrorControllers.cuh:500-540): returns the new step; updates updateStep, flags
+// termination and forced acceptance.
+struct ControlOut {
+  double dtNew;
+  int updateStep;
+  int terminate;
+  int forced;
+};
+
+template <class Math>
+ControlOut rkck45_control(int updateStep, int isFinite, double relErr, double dt,
+                          const bex_oracle_options& o) {
+  ControlOut c{0.0, updateStep, 0, 0};
+  double mult;
+  if (updateStep == 1)
+    mult = 0.8 * Math::pow(relErr, 1.0 / 5.0);
+  else
+    mult = 0.8 * Math::pow(relErr, 1.0 / 4.0);
+  if (!std::isfinite(mult)) isFinite = 0;
+  if (isFinite == 0) {
+    mult = o.TimeStepShrinkLimit;
+    c.updateStep = 0;
+    if (dt < (o.MinimumTimeStep * 1.01)) c.terminate = 1;
+  } else if (dt < (o.MinimumTimeStep * 1.01)) {
+    c.updateStep = 1;
+    c.forced = 1;
+  }
+  mult = Math::fmin(mult, o.TimeStepGrowLimit);
+  mult = Math::fmax(mult, o.TimeStepShrinkLimit);
+  double dtNew = dt * mult;
+  dtNew = Math::fmin(dtNew, o.MaximumTimeStep);
+  dtNew = Math::fmax(dtNew, o.MinimumTimeStep);
+  c.dtNew = dtNew;
+  return c;
+}
+
+// ---------------------------------------------------------------------------------------
 // One system over one Solve() (Solver.cuh:11-651 with TPB == UPS, SPB == 1).
 
 struct Ctx {
@@ -360,47 +469,9 @@ void solve_system(const Ctx& c, int sid) {
     st[BEX_STAT_ATTEMPTS]++;
 
     const int n2 = N * UD;
-    double* k1 = &K[0];
-    double* k2 = &K[n2];
-    double* k3 = &K[2 * n2];
-    double* k4 = &K[3 * n2];
-    double* k5 = &K[4 * n2];
-    double* k6 = &K[5 * n2];
     if (o.Algorithm == 1) {
-      // RKCK45 stepper (Steppers.cuh:153-424)
-      for (int i = 0; i < n2; i++) Xs[i] = X[i];
-      stage(0, t);
-      for (int i = 0; i < n2; i++) Xs[i] = X[i] + (k1[i] * (1.0 / 5.0)) * dt;
-      stage(1, t + (1.0 / 5.0) * dt);
-      for (int i = 0; i < n2; i++) Xs[i] = X[i] + (k1[i] * (3.0 / 40.0) + k2[i] * (9.0 / 40.0)) * dt;
-      stage(2, t + (3.0 / 10.0) * dt);
-      for (int i = 0; i < n2; i++)
-        Xs[i] = X[i] + (k1[i] * (3.0 / 10.0) + k2[i] * (-9.0 / 10.0) + k3[i] * (6.0 / 5.0)) * dt;
-      stage(3, t + (3.0 / 5.0) * dt);
-      for (int i = 0; i < n2; i++)
-        Xs[i] = X[i] + (k1[i] * (-11.0 / 54.0) + k2[i] * (5.0 / 2.0) + k3[i] * (-70.0 / 27.0) +
-                        k4[i] * (35.0 / 27.0)) *
-                           dt;
-      stage(4, t + dt);
-      for (int i = 0; i < n2; i++)
-        Xs[i] = X[i] + (k1[i] * (1631.0 / 55296.0) + k2[i] * (175.0 / 512.0) +
-                        k3[i] * (575.0 / 13824.0) + k4[i] * (44275.0 / 110592.0) +
-                        k5[i] * (253.0 / 4096.0)) *
-                           dt;
-      stage(5, t + (7.0 / 8.0) * dt);
-      for (int i = 0; i < n2; i++) {
-        Xn[i] = X[i] + (k1[i] * (37.0 / 378.0) + k3[i] * (250.0 / 621.0) +
-                        k4[i] * (125.0 / 594.0) + k6[i] * (512.0 / 1771.0)) *
-                           dt;
-        double e = k1[i] * (37.0 / 378.0 - 2825.0 / 27648.0) +
-                   k3[i] * (250.0 / 621.0 - 18575.0 / 48384.0) +
-                   k4[i] * (125.0 / 594.0 - 13525.0 / 55296.0) +
-                   k5[i] * (0.0 - 277.0 / 14336.0) + k6[i] * (512.0 / 1771.0 - 1.0 / 4.0);
-        Err[i] = dt * std::fabs(e) + 1e-18;
-        if (!std::isfinite(Xn[i]) || !std::isfinite(Err[i])) isFinite = 0;
-      }
-
-      // Error controller (ErrorControllers.cuh:453-542)
+      isFinite = rkck45_step(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), Err.data(), stage);
+      // Error controller reductions (ErrorControllers.cuh:477-497)
       double relErr = 1e30;
       for (int u = 0; u < N; u++) {
         double rr = 1e30;
@@ -416,46 +487,17 @@ void solve_system(const Ctx& c, int sid) {
         updateStep &= ru;
         if (rr < relErr) relErr = rr;  // atomicFMIN: NaN candidates leave the value unchanged
       }
-      double mult;
-      if (updateStep == 1)
-        mult = 0.8 * Math::pow(relErr, 1.0 / 5.0);
-      else
-        mult = 0.8 * Math::pow(relErr, 1.0 / 4.0);
-      if (!std::isfinite(mult)) isFinite = 0;
-      if (isFinite == 0) {
-        mult = o.TimeStepShrinkLimit;
-        updateStep = 0;
-        if (dt < (o.MinimumTimeStep * 1.01)) {
-          terminate = 1;
-          st[BEX_STAT_TERMINATED]++;
-        }
-      } else {
-        if (dt < (o.MinimumTimeStep * 1.01)) {
-          updateStep = 1;
-          st[BEX_STAT_FORCED_ACCEPTS]++;
-        }
+      ControlOut cc = rkck45_control<Math>(updateStep, isFinite, relErr, dt, o);
+      updateStep = cc.updateStep;
+      dtNew = cc.dtNew;
+      if (cc.terminate) {
+        terminate = 1;
+        st[BEX_STAT_TERMINATED]++;
       }
-      mult = Math::fmin(mult, o.TimeStepGrowLimit);
-      mult = Math::fmax(mult, o.TimeStepShrinkLimit);
-      dtNew = dt * mult;
-      dtNew = Math::fmin(dtNew, o.MaximumTimeStep);
-      dtNew = Math::fmax(dtNew, o.MinimumTimeStep);
+      if (cc.forced) st[BEX_STAT_FORCED_ACCEPTS]++;
     } else {
-      // Classical RK4 with a fixed step; the reference declares RK4 (Interface.cuh:33) and its
-      // controller (ErrorControllers.cuh:424-450) but has no per-block RK4 stepper.
-      for (int i = 0; i < n2; i++) Xs[i] = X[i];
-      stage(0, t);
-      for (int i = 0; i < n2; i++) Xs[i] = X[i] + k1[i] * (0.5 * dt);
-      stage(1, t + 0.5 * dt);
-      for (int i = 0; i < n2; i++) Xs[i] = X[i] + k2[i] * (0.5 * dt);
-      stage(2, t + 0.5 * dt);
-      for (int i = 0; i < n2; i++) Xs[i] = X[i] + k3[i] * dt;
-      stage(3, t + dt);
-      for (int i = 0; i < n2; i++) {
-        Xn[i] = X[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) * (dt * (1.0 / 6.0));
-        if (!std::isfinite(Xn[i])) isFinite = 0;
-      }
-      if (isFinite == 0) {
+      isFinite = rk4_step(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), stage);
+      if (isFinite == 0) {  // ErrorControllers.cuh:434-446
         terminate = 1;
         updateStep = 0;
         st[BEX_STAT_TERMINATED]++;
@@ -616,4 +658,35 @@ extern "C" double bex_oracle_sin(double x, int mode) {
 }
 extern "C" double bex_oracle_cos(double x, int mode) {
   return mode == BEX_ORACLE_LITERAL ? MathLiteral::cos(x) : MathCanonical::cos(x);
+}
+
+extern "C" int bex_oracle_rkck45_linear(double lambda, double y0, double h, double* y5,
+                                        double* err) {
+  // y' = lambda*y through the oracle's RKCK45 tableau (unit dimension 1).
+  double X[1] = {y0}, K[6], Xs[1], Xn[1], Err[1];
+  auto stage = [&](int s, double) { K[s] = lambda * Xs[0]; };
+  int ok = rkck45_step(1, X, 0.0, h, K, Xs, Xn, Err, stage);
+  *y5 = Xn[0];
+  *err = Err[0];
+  return ok;
+}
+
+extern "C" int bex_oracle_rk4_linear(double lambda, double y0, double h, double* y) {
+  double X[1] = {y0}, K[4], Xs[1], Xn[1];
+  auto stage = [&](int s, double) { K[s] = lambda * Xs[0]; };
+  int ok = rk4_step(1, X, 0.0, h, K, Xs, Xn, stage);
+  *y = Xn[0];
+  return ok;
+}
+
+extern "C" void bex_oracle_control(int updateStep, int isFinite, double relErr, double dt,
+                                   const bex_oracle_options* o, int mode, double* dtNew,
+                                   int* updateOut, int* terminate, int* forced) {
+  ControlOut c = (mode == BEX_ORACLE_LITERAL)
+                     ? rkck45_control<MathLiteral>(updateStep, isFinite, relErr, dt, *o)
+                     : rkck45_control<MathCanonical>(updateStep, isFinite, relErr, dt, *o);
+  *dtNew = c.dtNew;
+  *updateOut = c.updateStep;
+  *terminate = c.terminate;
+  *forced = c.forced;
 }

@@ -103,6 +103,15 @@ int bex_oracle_bicg(int N, const double* M, const double* x1, const double* rD,
 void bex_oracle_km_rhs(const double* X, double T, const double* uPAR, int mode, double* F,
                        double* CPT, double* CPF);
 
+/* The oracle's RKCK45 / RK4 tableaux applied to y' = lambda*y for one step (tests). */
+int bex_oracle_rkck45_linear(double lambda, double y0, double h, double* y5, double* err);
+int bex_oracle_rk4_linear(double lambda, double y0, double h, double* y);
+
+/* The RKCK45 step-size controller after its reductions (tests). */
+void bex_oracle_control(int updateStep, int isFinite, double relErr, double dt,
+                        const bex_oracle_options* o, int mode, double* dtNew, int* updateOut,
+                        int* terminate, int* forced);
+
 /* Deterministic elementary functions, exposed so tests can measure their accuracy. */
 double bex_oracle_pow(double x, double y, int mode);
 void bex_oracle_sincospi(double x, int mode, double* s, double* c);

@@ -100,6 +100,14 @@ def lib():
         L.bex_oracle_sin.restype = ctypes.c_double
         L.bex_oracle_cos.argtypes = [ctypes.c_double, ctypes.c_int]
         L.bex_oracle_cos.restype = ctypes.c_double
+        L.bex_oracle_rkck45_linear.argtypes = [ctypes.c_double] * 3 + [_DP, _DP]
+        L.bex_oracle_rkck45_linear.restype = ctypes.c_int
+        L.bex_oracle_rk4_linear.argtypes = [ctypes.c_double] * 3 + [_DP]
+        L.bex_oracle_rk4_linear.restype = ctypes.c_int
+        L.bex_oracle_control.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.POINTER(Options), ctypes.c_int,
+                                         _DP, _IP, _IP, _IP]
+        L.bex_oracle_control.restype = None
         _lib = L
     return _lib
 
@@ -226,3 +234,50 @@ def km_rhs(X, T, upar, mode=CANONICAL):
     cpf = np.zeros(1)
     lib().bex_oracle_km_rhs(_dp(X), float(T), _dp(upar), mode, _dp(F), _dp(cpt), _dp(cpf))
     return F, cpt[0], cpf[0]
+
+
+def rkck45_linear(lam, y0, h):
+    y = ctypes.c_double()
+    e = ctypes.c_double()
+    ok = lib().bex_oracle_rkck45_linear(lam, y0, h, ctypes.byref(y), ctypes.byref(e))
+    return y.value, e.value, ok
+
+
+def rk4_linear(lam, y0, h):
+    y = ctypes.c_double()
+    ok = lib().bex_oracle_rk4_linear(lam, y0, h, ctypes.byref(y))
+    return y.value, ok
+
+
+def control(update_step, is_finite, rel_err, dt, mode=CANONICAL, MinimumTimeStep=1e-12,
+            MaximumTimeStep=1e6, TimeStepGrowLimit=5.0, TimeStepShrinkLimit=0.1):
+    opt = Options()
+    opt.MinimumTimeStep = MinimumTimeStep
+    opt.MaximumTimeStep = MaximumTimeStep
+    opt.TimeStepGrowLimit = TimeStepGrowLimit
+    opt.TimeStepShrinkLimit = TimeStepShrinkLimit
+    dt_new = ctypes.c_double()
+    upd, term, forced = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().bex_oracle_control(update_step, is_finite, rel_err, dt, ctypes.byref(opt), mode,
+                             ctypes.byref(dt_new), ctypes.byref(upd), ctypes.byref(term),
+                             ctypes.byref(forced))
+    return dt_new.value, upd.value, term.value, forced.value
+
+
+def pow_(x, y, mode=CANONICAL):
+    return lib().bex_oracle_pow(x, y, mode)
+
+
+def sincospi(x, mode=CANONICAL):
+    s = ctypes.c_double()
+    c = ctypes.c_double()
+    lib().bex_oracle_sincospi(x, mode, ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def sin_(x, mode=CANONICAL):
+    return lib().bex_oracle_sin(x, mode)
+
+
+def cos_(x, mode=CANONICAL):
+    return lib().bex_oracle_cos(x, mode)
