@@ -67,6 +67,9 @@ _SIGNATURES = {
     "bex_reset_stats": ([ctypes.c_void_p], ctypes.c_int),
     "bex_timer_start": ([ctypes.c_void_p], ctypes.c_int),
     "bex_timer_stop": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_enable_timeline": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_get_timeline": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                          ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
 }
 

@@ -31,6 +31,7 @@ struct bex_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t event = nullptr;
+  hipEvent_t timer0 = nullptr, timer1 = nullptr;
   bex_problem_desc d{};
   // Thread configuration of the reference layout (Interface.cuh:386-391).
   int LogicalThreadsPerBlock = 0, ThreadPaddingPerBlock = 0, GridSize = 0;
@@ -43,17 +44,20 @@ struct bex_solver {
       DenseOutputIndex, DenseOutputTimeInstances, DenseOutputStates;
   double* h_CouplingOriginal = nullptr;  // full [c][N][N], host only (Interface.cuh:1076)
   double* d_CouplingFull = nullptr;      // full [c][N][N] expanded from the compact storage
+  double* d_CouplingFullT = nullptr;     // transpose of coupling 0
   double* h_tol = nullptr;               // [2*UD + NE] rtol, atol, event tolerance
   int* h_dir = nullptr;                  // [NE]
   double* d_tol = nullptr;
   int* d_dir = nullptr;
   long long* d_stats = nullptr;
+  long long* d_timeline = nullptr;  // [NS][2] start/end wall-clock ticks, when enabled
 
   double InitialTimeStep = 1e-2, MaximumTimeStep = 1.0e6, MinimumTimeStep = 1.0e-12;
   double TimeStepGrowLimit = 5.0, TimeStepShrinkLimit = 0.1, DenseOutputMinimumTimeStep = 0.0;
   int ActiveSystems = 0, DenseOutputSaveFrequency = 1;
   double BiCGRhoTolerance = 1e-6, BiCGResidualTolerance = 1e-6;
   int BiCGMaxIterations = 19;
+  int KernelLayout = BEX_LAYOUT_AUTO;
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
   std::string err;
 };
@@ -90,18 +94,33 @@ int bound(bex_solver* s, const char* fn, const char* what, long long v, long lon
 
 using KernelFn = void (*)(BexKernelArgs);
 
-template <int N>
-KernelFn pick(int alg) {
-  return alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<N, bexk::kRK4>
-                        : (KernelFn)bexk::solve_kernel<N, bexk::kRKCK45>;
+struct KernelChoice {
+  KernelFn fn;
+  size_t lds;
+  int systems_per_block;
+  int block;
+};
+
+template <class L>
+KernelChoice choice(int alg) {
+  KernelFn fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4>
+                               : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45>;
+  return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
 }
 
-bool kernel_for(int ups, int alg, KernelFn* fn, size_t* lds, int* spb) {
+// layout: BEX_LAYOUT_AUTO picks the role-pair layout for 32 units and the group layout
+// otherwise.
+bool kernel_for(int ups, int alg, int layout, KernelChoice* k) {
+  if (ups == 32 && layout != BEX_LAYOUT_GROUP) {
+    *k = choice<bexk::RolePairLayout32>(alg);
+    return true;
+  }
+  if (layout == BEX_LAYOUT_ROLE_PAIR) return false;
   switch (ups) {
-    case 8: *fn = pick<8>(alg); *lds = bexk::lds_bytes<8>(); *spb = bexk::Geometry<8>::SPB; return true;
-    case 16: *fn = pick<16>(alg); *lds = bexk::lds_bytes<16>(); *spb = bexk::Geometry<16>::SPB; return true;
-    case 32: *fn = pick<32>(alg); *lds = bexk::lds_bytes<32>(); *spb = bexk::Geometry<32>::SPB; return true;
-    case 64: *fn = pick<64>(alg); *lds = bexk::lds_bytes<64>(); *spb = bexk::Geometry<64>::SPB; return true;
+    case 8: *k = choice<bexk::GroupLayout<8>>(alg); return true;
+    case 16: *k = choice<bexk::GroupLayout<16>>(alg); return true;
+    case 32: *k = choice<bexk::GroupLayout<32>>(alg); return true;
+    case 64: *k = choice<bexk::GroupLayout<64>>(alg); return true;
     default: return false;
   }
 }
@@ -184,6 +203,16 @@ int upload_coupling(bex_solver* s) {
   hipError_t e = hipMemcpyAsync(s->d_CouplingFull, full, sizeof(double) * (size_t)NC * N * N,
                                 hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  // Transpose of coupling 0 (the one the implicit solve uses), for coalesced row loads of M^T.
+  for (int col = 0; col < N && e == hipSuccess; col++)
+    for (int row = 0; row < N; row++) {
+      long long k = compact_index(s, 0, col, row);
+      full[row + (long long)col * N] = (k >= 0) ? static_cast<double*>(s->CouplingMatrix.h)[k] : 0.0;
+    }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->d_CouplingFullT, full, sizeof(double) * (size_t)N * N,
+                       hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   hipHostFree(full);
   if (e != hipSuccess) return fail(s, BEX_E_HIP, std::string("coupling upload: ") + hipGetErrorString(e));
   return BEX_OK;
@@ -245,10 +274,8 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   if (d.NS <= 0 || d.TPB <= 0 || d.SPB <= 0 || d.NC <= 0 || d.NDO < 0 || d.CBW < 0 ||
       (d.CCI != 0 && d.CCI != 1) || (d.Algorithm != BEX_RK4 && d.Algorithm != BEX_RKCK45))
     return fail(nullptr, BEX_E_ARGUMENT, "bex_create: invalid problem description");
-  KernelFn fn;
-  size_t lds;
-  int spb;
-  if (!kernel_for(d.UPS, d.Algorithm, &fn, &lds, &spb))
+  KernelChoice kc;
+  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
                 "bex_create: units per system must be 8, 16, 32 or 64 in this build");
 
@@ -258,6 +285,8 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&s->event);
+  if (e == hipSuccess) e = hipEventCreate(&s->timer0);
+  if (e == hipSuccess) e = hipEventCreate(&s->timer1);
   if (e != hipSuccess) {
     g_create_error = std::string("bex_create: ") + hipGetErrorString(e);
     delete s;
@@ -324,6 +353,7 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   ok = ok && hipHostMalloc((void**)&s->h_CouplingOriginal, sizeof(double) * (size_t)d.NC * N * N,
                            hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_CouplingFull, sizeof(double) * (size_t)d.NC * N * N) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_CouplingFullT, sizeof(double) * (size_t)N * N) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&s->h_tol, sizeof(double) * ntol, hipHostMallocDefault) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&s->h_dir, sizeof(int) * (d.NE > 0 ? d.NE : 1), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_tol, sizeof(double) * ntol) == hipSuccess;
@@ -375,12 +405,16 @@ int bex_destroy(bex_solver* s) {
   }
   if (s->h_CouplingOriginal) hipHostFree(s->h_CouplingOriginal);
   if (s->d_CouplingFull) hipFree(s->d_CouplingFull);
+  if (s->d_CouplingFullT) hipFree(s->d_CouplingFullT);
   if (s->h_tol) hipHostFree(s->h_tol);
   if (s->h_dir) hipHostFree(s->h_dir);
   if (s->d_tol) hipFree(s->d_tol);
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
+  if (s->d_timeline) hipFree(s->d_timeline);
   if (s->event) hipEventDestroy(s->event);
+  if (s->timer0) hipEventDestroy(s->timer0);
+  if (s->timer1) hipEventDestroy(s->timer1);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return BEX_OK;
@@ -402,6 +436,13 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_BiCGRhoTolerance: s->BiCGRhoTolerance = v; break;
     case BEX_BiCGResidualTolerance: s->BiCGResidualTolerance = v; break;
     case BEX_BiCGMaxIterations: s->BiCGMaxIterations = (int)v; break;
+    case BEX_KernelLayout: {
+      KernelChoice kc;
+      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, &kc))
+        return fail(s, BEX_E_UNSUPPORTED, "SolverOption: no kernel layout for this unit count");
+      s->KernelLayout = (int)v;
+      break;
+    }
     default:
       return fail(s, BEX_E_OPTION,
                   "SolverOption: this option needs 2 input arguments or is not applicable");
@@ -696,11 +737,9 @@ int bex_sync_d2h(bex_solver* s, int var) {
 int bex_solve(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
-  KernelFn fn;
-  size_t lds;
-  int spb;
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, &fn, &lds, &spb))
-    return fail(s, BEX_E_UNSUPPORTED, "Solve: unsupported units per system");
+  KernelChoice kc;
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, &kc))
+    return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count and layout");
   const int ND = s->d.UD;
   BexKernelArgs a{};
   a.NS = s->d.NS;
@@ -720,6 +759,7 @@ int bex_solve(bex_solver* s) {
   a.SystemAccessories = (double*)s->SystemAccessories.d;
   a.IntegerSystemAccessories = (int*)s->IntegerSystemAccessories.d;
   a.CouplingMatrix = s->d_CouplingFull;
+  a.CouplingMatrixT = s->d_CouplingFullT;
   a.DenseOutputIndex = (int*)s->DenseOutputIndex.d;
   a.DenseOutputTimeInstances = (double*)s->DenseOutputTimeInstances.d;
   a.DenseOutputStates = (double*)s->DenseOutputStates.d;
@@ -728,6 +768,7 @@ int bex_solve(bex_solver* s) {
   a.EventTolerance = s->d_tol + 2 * ND;
   a.EventDirection = s->d_dir;
   a.Stats = s->d_stats;
+  a.Timeline = s->d_timeline;
   a.InitialTimeStep = s->InitialTimeStep;
   a.MaximumTimeStep = s->MaximumTimeStep;
   a.MinimumTimeStep = s->MinimumTimeStep;
@@ -742,8 +783,8 @@ int bex_solve(bex_solver* s) {
   a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
   if (nsys <= 0) return BEX_OK;
-  dim3 grid((nsys + spb - 1) / spb), block(bexk::kBlock);
-  hipLaunchKernelGGL(fn, grid, block, lds, s->stream, a);
+  dim3 grid((nsys + kc.systems_per_block - 1) / kc.systems_per_block), block(kc.block);
+  hipLaunchKernelGGL(kc.fn, grid, block, kc.lds, s->stream, a);
   BEX_HIP(s, hipGetLastError());
   return BEX_OK;
 }
@@ -775,6 +816,51 @@ int bex_get_stats(bex_solver* s, long long* out) {
   BEX_HIP(s, hipMemcpyAsync(out, s->d_stats, sizeof(long long) * (size_t)s->d.NS * BEX_NSTAT,
                             hipMemcpyDeviceToHost, s->stream));
   BEX_HIP(s, hipStreamSynchronize(s->stream));
+  return BEX_OK;
+}
+
+int bex_timer_start(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipEventRecord(s->timer0, s->stream));
+  return BEX_OK;
+}
+
+int bex_timer_stop(bex_solver* s, double* elapsed_ms) {
+  if (!s || !elapsed_ms) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipEventRecord(s->timer1, s->stream));
+  BEX_HIP(s, hipEventSynchronize(s->timer1));
+  float ms = 0.0f;
+  BEX_HIP(s, hipEventElapsedTime(&ms, s->timer0, s->timer1));
+  *elapsed_ms = ms;
+  return BEX_OK;
+}
+
+int bex_enable_timeline(bex_solver* s, int on) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  if (on && !s->d_timeline) {
+    BEX_HIP(s, hipMalloc((void**)&s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS));
+    BEX_HIP(s, hipMemsetAsync(s->d_timeline, 0, sizeof(long long) * 2 * (size_t)s->d.NS, s->stream));
+  } else if (!on && s->d_timeline) {
+    BEX_HIP(s, hipStreamSynchronize(s->stream));
+    BEX_HIP(s, hipFree(s->d_timeline));
+    s->d_timeline = nullptr;
+  }
+  return BEX_OK;
+}
+
+int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second) {
+  if (!s || !out || !ticks_per_second) return BEX_E_ARGUMENT;
+  if (!s->d_timeline) return fail(s, BEX_E_ARGUMENT, "bex_get_timeline: timeline not enabled");
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipMemcpyAsync(out, s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS,
+                            hipMemcpyDeviceToHost, s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  int khz = 0;
+  BEX_HIP(s, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device));
+  *ticks_per_second = 1e3 * khz;
   return BEX_OK;
 }
 

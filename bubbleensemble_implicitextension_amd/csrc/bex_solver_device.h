@@ -1,20 +1,25 @@
-// gfx950 solver kernel: adaptive RKCK45 (or fixed-step RK4) integration of coupled
+// gfx950 solver kernels: adaptive RKCK45 (or fixed-step RK4) integration of coupled
 // Keller-Miksis ensembles with an in-wave BiCG solve of the implicit coupling.
 //
-// Work decomposition (MI355X, wave64):
-//   * one lane = one unit (bubble), one group of N consecutive lanes = one system;
-//     a wave holds 64/N systems, a 256-thread workgroup holds 4 waves;
-//   * the coupling matrix (coupling 0, shared by all systems as in the reference) and its
-//     transpose live in LDS, loaded once per workgroup;
-//   * every system-level reduction (dot products, residual max, step acceptance, step-size
-//     minimum, event step) is a butterfly inside the lane group: no workgroup barrier is
-//     executed after the initial matrix load, so systems in one workgroup never wait for
-//     each other;
-//   * vectors that every unit needs (the BiCG search directions) are exchanged through a
-//     per-system LDS slot, read back as broadcast ds_read_b128 of (u_j, w_j) pairs.
+// The time loop (Solver.cuh:329-572 semantics) is written once and parametrised by a lane
+// layout that owns the coupling matrix and implements the BiCG solve and the system-level
+// reductions:
 //
-// Arithmetic order is fixed (explicit fma, -ffp-contract=off) and identical to the CPU
-// oracle's canonical mode, so the kernel reproduces it bit for bit.
+//   RolePairLayout32  N = 32, one system per wave. Lanes 0-31 ("primal") hold unit i and row i
+//                     of M; lanes 32-63 ("shadow") hold the same unit and row i of M^T. One
+//                     BiCG matrix-vector step computes A p on the primal half and A^T p~ on the
+//                     shadow half at once: 32 v_fmac_f64 with a DPP row_newbcast operand and
+//                     two v_permlane16_swap, no LDS traffic. Dot products pair the halves with
+//                     v_permlane32_swap and reduce with the reference's halving tree through
+//                     permlane16/row_shl DPP; every system-level decision is wave-uniform.
+//   GroupLayout<N>    N = 8, 16, 64: lane = unit, a wave holds 64/N systems, M and M^T live in
+//                     LDS (one copy per 4-wave workgroup) and the search directions are
+//                     exchanged through per-system LDS slots.
+//
+// Arithmetic order is fixed (explicit fma, -ffp-contract=off) and identical to the CPU oracle's
+// canonical mode: sequential fma accumulation over columns for every matrix-vector product,
+// the reference's halving-tree order for every dot product. The kernels reproduce the oracle
+// bit for bit.
 //
 // Reference: Solver.cuh = SourceCodes/CoupledSystmes_PerBlock_Solver.cuh,
 // Steppers.cuh = SourceCodes/CoupledSystems_PerBlock_ExplicitRungeKutta_Steppers.cuh,
@@ -31,20 +36,17 @@
 #endif
 #include BEX_SYSTEM_DEFINITION
 
+// Occupancy target of the role-pair kernel: 3 waves per SIMD leaves 168 VGPRs per lane for the
+// 64 that hold the coupling-matrix row.
+#ifndef BEX_ROLEPAIR_WAVES_PER_EU
+#define BEX_ROLEPAIR_WAVES_PER_EU 3
+#endif
+
 namespace bexk {
 
 constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWave * kWavesPerBlock;
 
 enum Algorithm { kRK4 = 0, kRKCK45 = 1 };
-
-template <int N>
-struct Geometry {
-  static_assert(N == 8 || N == 16 || N == 32 || N == 64, "units per system: 8, 16, 32 or 64");
-  static constexpr int SPW = kWave / N;               // systems per wave
-  static constexpr int SPB = SPW * kWavesPerBlock;    // systems per workgroup
-};
 
 // System-scope values the user hooks receive by reference (the reference keeps them in
 // shared memory, Solver.cuh:51-74).
@@ -59,13 +61,7 @@ struct SysShared {
   int UDT;
   int DOIDX;
 };
-
-template <int N>
-__host__ __device__ constexpr size_t lds_bytes() {
-  using D = BexSystemDims;
-  return sizeof(double) * 2 * N * N + sizeof(double2) * Geometry<N>::SPB * N +
-         sizeof(SysShared<D>) * Geometry<N>::SPB + sizeof(int) * Geometry<N>::SPB * BEX_NSTAT;
-}
+using Sys = SysShared<BexSystemDims>;
 
 // Orders LDS traffic between the lanes of one wave: LDS operations of a wave complete in
 // issue order, so a compiler-level barrier is all that is needed.
@@ -75,49 +71,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sum over the N lanes of a group. The butterfly leaves every lane with
-// ((v0+v[N/2]) + ...) in exactly the association of the reference's halving tree
-// (Steppers.cuh:733-742); IEEE addition is commutative, so all lanes agree bitwise.
-template <int N>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int h = N / 2; h >= 1; h >>= 1) v = v + __shfl_xor(v, h, kWave);
-  return v;
-}
-
-// Maximum with the C fmax NaN rule (Steppers.cuh:776-805 uses fmax).
-template <int N>
-__device__ __forceinline__ double group_fmax(double v) {
-#pragma unroll
-  for (int h = N / 2; h >= 1; h >>= 1) v = bex::fmax_d(v, __shfl_xor(v, h, kWave));
-  return v;
-}
-
-// Minimum where a NaN candidate never replaces a value (MathFunction.cuh:54-64 atomicFMIN).
-template <int N>
-__device__ __forceinline__ double group_min_keep(double v) {
-#pragma unroll
-  for (int h = N / 2; h >= 1; h >>= 1) {
-    double o = __shfl_xor(v, h, kWave);
-    v = (o < v) ? o : v;
-  }
-  return v;
-}
-
-template <int N>
-__device__ __forceinline__ unsigned long long group_mask(int grp) {
-  return (N == 64) ? ~0ull : (((1ull << N) - 1ull) << (grp * N));
-}
-
-template <int N>
-__device__ __forceinline__ bool group_all(bool p, int grp) {
-  unsigned long long m = group_mask<N>(grp);
-  return (__ballot(p) & m) == m;
-}
-
-template <int N>
-__device__ __forceinline__ bool group_any(bool p, int grp) {
-  return (__ballot(p) & group_mask<N>(grp)) != 0ull;
+__device__ __forceinline__ double uniform(double v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double((int)hi, (int)lo);
 }
 
 struct BicgOut {
@@ -126,92 +83,343 @@ struct BicgOut {
   bool maxit;
 };
 
-// One right-hand-side evaluation with the implicit coupling solve for unit uid
-// (Steppers.cuh:431-562). On return F[1] holds the coupled acceleration.
-//   A x = b,  A = I + diag(rD) M diag(x1^2),  b = xu + rD * (M CPT),  x0 = xu.
+// =========================================================================================
+// GroupLayout<N>: lane = unit, 64/N systems per wave, 4 waves per workgroup.
+
 template <int N>
-__device__ __forceinline__ BicgOut stage_rhs(double* F, double* Xs, double Ts, int sid, int uid,
-                                             double* upar, SysShared<BexSystemDims>& S,
-                                             double* uacc, int* iuacc, const BexKernelArgs& a,
-                                             const double* sM,
-                                             const double* sMT, double2* vec) {
-  double cpt, cpf;
-  CoupledSystems_PerBlock_OdeFunction<double>(sid, uid, F, Xs, Ts, upar, S.sPAR,
-                                              a.GlobalParameters, a.IntegerGlobalParameters,
-                                              uacc, iuacc, S.sACC, S.isACC, &cpt, &cpf);
-  const double rD = cpf;
-  const double x1sq = Xs[0] * Xs[0];  // Steppers.cuh:491
-  const double xu = F[1];             // Steppers.cuh:492
+struct GroupLayout {
+  static_assert(N == 8 || N == 16 || N == 32 || N == 64, "units per system: 8, 16, 32 or 64");
+  static constexpr int kUnits = N;
+  static constexpr int kSystemsPerWave = kWave / N;
+  static constexpr int kWaves = 4;
+  static constexpr int kBlock = kWave * kWaves;
+  static constexpr int kSystemsPerBlock = kSystemsPerWave * kWaves;
+  static constexpr int kWavesPerEU = 4;  // <= 128 VGPRs
 
-  // b = rD * (M CPT) + xu   (Steppers.cuh:497-498, :576-584)
-  wave_sync();
-  vec[uid].x = cpt;
-  wave_sync();
-  double acc = 0.0;
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * 2 * N * N + sizeof(double2) * kSystemsPerBlock * N +
+           sizeof(Sys) * kSystemsPerBlock + sizeof(int) * kSystemsPerBlock * BEX_NSTAT;
+  }
+
+  int lane, grp, uid_, lsys;
+  const double* sM;
+  const double* sMT;
+  double2* vec;
+  Sys* sys;
+  int* stat;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
+    const int tid = threadIdx.x;
+    lane = tid & (kWave - 1);
+    grp = lane / N;
+    uid_ = lane % N;
+    lsys = (tid / kWave) * kSystemsPerWave + grp;
+    double* m = reinterpret_cast<double*>(smem);
+    double* mt = m + N * N;
+    double2* v = reinterpret_cast<double2*>(mt + N * N);
+    Sys* s = reinterpret_cast<Sys*>(v + kSystemsPerBlock * N);
+    int* st = reinterpret_cast<int*>(s + kSystemsPerBlock);
+    for (int k = tid; k < N * N; k += kBlock) {
+      m[k] = a.CouplingMatrix[k];
+      mt[k] = a.CouplingMatrixT[k];
+    }
+    for (int k = tid; k < kSystemsPerBlock * BEX_NSTAT; k += kBlock) st[k] = 0;
+    __syncthreads();  // the only workgroup barrier
+    sM = m;
+    sMT = mt;
+    vec = v + lsys * N;
+    sys = s + lsys;
+    stat = st + lsys * BEX_NSTAT;
+  }
+
+  __device__ __forceinline__ int system_in_launch() const {
+    return blockIdx.x * kSystemsPerBlock + lsys;
+  }
+  __device__ __forceinline__ int uid() const { return uid_; }
+  __device__ __forceinline__ bool primal() const { return true; }
+
+  __device__ __forceinline__ unsigned long long mask() const {
+    return (N == 64) ? ~0ull : (((1ull << N) - 1ull) << (grp * N));
+  }
+  __device__ __forceinline__ bool all(bool p) const { return (__ballot(p) & mask()) == mask(); }
+  __device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask()) != 0ull; }
+
+  // Halving-tree sum (Steppers.cuh:733-742) as an xor butterfly: every lane ends with the
+  // tree's association (IEEE addition is commutative).
+  __device__ __forceinline__ double sum(double v) const {
+#pragma unroll
+    for (int h = N / 2; h >= 1; h >>= 1) v = v + __shfl_xor(v, h, kWave);
+    return v;
+  }
+  __device__ __forceinline__ double fmax_all(double v) const {
+#pragma unroll
+    for (int h = N / 2; h >= 1; h >>= 1) v = bex::fmax_d(v, __shfl_xor(v, h, kWave));
+    return v;
+  }
+  // Minimum where a NaN candidate never replaces a value (MathFunction.cuh:54-64 atomicFMIN).
+  __device__ __forceinline__ double min_keep(double v) const {
+#pragma unroll
+    for (int h = N / 2; h >= 1; h >>= 1) {
+      const double o = __shfl_xor(v, h, kWave);
+      v = (o < v) ? o : v;
+    }
+    return v;
+  }
+
+  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
+  // (Steppers.cuh:489-561). Returns x in F[1].
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    wave_sync();
+    vec[uid_].x = cpt;
+    wave_sync();
+    double acc = 0.0;
 #pragma unroll 8
-  for (int j = 0; j < N; j++) acc = fma(sM[uid + j * N], vec[j].x, acc);
-  const double b = fma(rD, acc, xu);
-
-  // r0 = b - A xu   (Steppers.cuh:502-504, :647-666)
-  wave_sync();
-  vec[uid].x = x1sq * xu;
-  wave_sync();
-  acc = 0.0;
+    for (int j = 0; j < N; j++) acc = fma(sM[uid_ + j * N], vec[j].x, acc);
+    const double b = fma(rD, acc, xu);
+    wave_sync();
+    vec[uid_].x = x1sq * xu;
+    wave_sync();
+    acc = 0.0;
 #pragma unroll 8
-  for (int j = 0; j < N; j++) acc = fma(sM[uid + j * N], vec[j].x, acc);
-  const double ax = fma(rD, acc, xu);
-  double r = -ax + b;
-  double rt = r;
-
-  double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
-  BicgOut out{0, false, true};
-  for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {  // Steppers.cuh:508
-    const double rho1 = group_sum<N>(r * rt);                 // :511-513
-    if (iter == 1) {                                          // :517-523
-      p = r;
-      pt = rt;
-      if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
-        out.rho_exit = true;
+    for (int j = 0; j < N; j++) acc = fma(sM[uid_ + j * N], vec[j].x, acc);
+    const double ax = fma(rD, acc, xu);
+    double r = -ax + b;
+    double rt = r;
+    double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
+    BicgOut out{0, false, true};
+    for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
+      const double rho1 = sum(r * rt);
+      if (iter == 1) {
+        p = r;
+        pt = rt;
+        if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
+          out.rho_exit = true;
+          out.maxit = false;
+          break;
+        }
+      } else {
+        const double beta = rho1 / rho2;
+        p = r + beta * p;
+        pt = rt + beta * pt;
+      }
+      wave_sync();
+      vec[uid_] = make_double2(x1sq * p, rD * pt);
+      wave_sync();
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll 8
+      for (int j = 0; j < N; j++) {
+        const double2 v = vec[j];
+        a1 = fma(sM[uid_ + j * N], v.x, a1);
+        a2 = fma(sMT[uid_ + j * N], v.y, a2);
+      }
+      const double q = fma(rD, a1, p);
+      const double qt = fma(x1sq, a2, pt);
+      out.iterations++;
+      const double alpha = rho1 / sum(pt * q);
+      x = x + alpha * p;
+      r = r - alpha * q;
+      rt = rt - alpha * qt;
+      rho2 = rho1;
+      if (fmax_all(bex::fabs_d(r)) < a.BiCGResidualTolerance) {
         out.maxit = false;
         break;
       }
-    } else {                                                  // :526-531
-      const double beta = rho1 / rho2;
-      p = r + beta * p;
-      pt = rt + beta * pt;
     }
-    // q = A p, qt = A^T pt   (:536, :669-706) as scaled products with M and M^T.
-    wave_sync();
-    vec[uid] = make_double2(x1sq * p, rD * pt);
-    wave_sync();
-    double a1 = 0.0, a2 = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < N; j++) {
-      const double2 v = vec[j];
-      a1 = fma(sM[uid + j * N], v.x, a1);
-      a2 = fma(sMT[uid + j * N], v.y, a2);
-    }
-    const double q = fma(rD, a1, p);
-    const double qt = fma(x1sq, a2, pt);
-    out.iterations++;
-    const double alpha = rho1 / group_sum<N>(pt * q);        // :539-541
-    x = x + alpha * p;                                        // :545-547
-    r = r - alpha * q;
-    rt = rt - alpha * qt;
-    rho2 = rho1;                                              // :551-552
-    if (group_fmax<N>(bex::fabs_d(r)) < a.BiCGResidualTolerance) {  // :556-559
-      out.maxit = false;
-      break;
-    }
+    F[1] = x;
+    return out;
   }
-  F[1] = x;
-  return out;
+};
+
+// =========================================================================================
+// RolePairLayout32: one N = 32 system per wave, primal and shadow halves.
+
+// acc = fma(z(lane k of the lane's 16-lane row), m[k], acc) for k = 0..15, in order. The
+// s_nop covers the DPP read-after-VALU-write hazard of z.
+__device__ __forceinline__ void fmac_row16(double& acc, double z, const double* m) {
+  asm volatile(
+      "s_nop 1\n"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %14 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %15 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %16 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %1, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc)
+      : "v"(z), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]),
+        "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]),
+        "v"(m[14]), "v"(m[15]));
 }
 
-template <int N, int ALG>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void solve_kernel(const BexKernelArgs a) {
+// Row exchange inside each 32-lane half: returns the value of the half's even row and of its
+// odd row at the lane's position in the row, i.e. z[c] and z[16 + c] of the lane's half.
+__device__ __forceinline__ void rows_of_half(double z, double* even, double* odd) {
+  const unsigned lo = __double2loint(z), hi = __double2hiint(z);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  *even = __hiloint2double((int)h[0], (int)l[0]);
+  *odd = __hiloint2double((int)h[1], (int)l[1]);
+}
+
+// Value of the lower half and of the upper half at lanes i and i + 32.
+__device__ __forceinline__ void halves(double z, double* lower, double* upper) {
+  const unsigned lo = __double2loint(z), hi = __double2hiint(z);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  *lower = __hiloint2double((int)h[0], (int)l[0]);
+  *upper = __hiloint2double((int)h[1], (int)l[1]);
+}
+
+// Halving tree over lanes 0..31 (Steppers.cuh:733-742): s[i] = op(s[i], s[i + h]) for
+// h = 16, 8, 4, 2, 1; the result is lane 0's, broadcast to the wave.
+template <class Op>
+__device__ __forceinline__ double tree32(double s, Op op) {
+  double even, odd;
+  rows_of_half(s, &even, &odd);
+  s = op(s, odd);
+  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x108, 0xf, 0xf, true));  // row_shl:8
+  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x104, 0xf, 0xf, true));  // row_shl:4
+  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x102, 0xf, 0xf, true));  // row_shl:2
+  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x101, 0xf, 0xf, true));  // row_shl:1
+  return uniform(s);
+}
+
+struct AddOp {
+  __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
+};
+struct FmaxOp {
+  __device__ __forceinline__ double operator()(double a, double b) const {
+    return bex::fmax_d(a, b);
+  }
+};
+
+struct RolePairLayout32 {
+  static constexpr int kUnits = 32;
+  static constexpr int kSystemsPerWave = 1;
+  static constexpr int kWaves = 1;
+  static constexpr int kBlock = kWave;
+  static constexpr int kSystemsPerBlock = 1;
+  static constexpr int kWavesPerEU = BEX_ROLEPAIR_WAVES_PER_EU;
+
+  static constexpr size_t lds_bytes() { return sizeof(Sys) + sizeof(int) * BEX_NSTAT; }
+
+  int lane;
+  bool primal_;
+  double mrow[32];  // row i of M (primal) or of M^T (shadow)
+  Sys* sys;
+  int* stat;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
+    lane = threadIdx.x;
+    primal_ = lane < 32;
+    const int i = lane & 31;
+    const double* src = primal_ ? a.CouplingMatrix : a.CouplingMatrixT;
+#pragma unroll
+    for (int j = 0; j < 32; j++) mrow[j] = src[i + j * 32];
+    sys = reinterpret_cast<Sys*>(smem);
+    stat = reinterpret_cast<int*>(sys + 1);
+    if (lane < BEX_NSTAT) stat[lane] = 0;
+    wave_sync();
+  }
+
+  __device__ __forceinline__ int system_in_launch() const { return blockIdx.x; }
+  __device__ __forceinline__ int uid() const { return lane & 31; }
+  __device__ __forceinline__ bool primal() const { return primal_; }
+
+  // Both halves hold identical per-unit values, so wave-wide AND/OR/min equal the system's.
+  __device__ __forceinline__ bool all(bool p) const { return __ballot(!p) == 0ull; }
+  __device__ __forceinline__ bool any(bool p) const { return __ballot(p) != 0ull; }
+  __device__ __forceinline__ double min_keep(double v) const {
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+      const double o = __shfl_xor(v, h, kWave);
+      v = (o < v) ? o : v;
+    }
+    return uniform(v);
+  }
+
+  // acc_i = sum_j mrow[j] * z_j over j = 0..31 (sequential fma), z from the lane's half.
+  __device__ __forceinline__ double matvec(double z) const {
+    double even, odd;
+    rows_of_half(z, &even, &odd);
+    double acc = 0.0;
+    fmac_row16(acc, even, &mrow[0]);
+    fmac_row16(acc, odd, &mrow[16]);
+    return acc;
+  }
+
+  // sum_i a_i b_i (tree order) for a on the primal half and b on the shadow half of v.
+  __device__ __forceinline__ double dot_halves(double v) const {
+    double lo, up;
+    halves(v, &lo, &up);
+    return tree32(lo * up, AddOp());
+  }
+
+  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
+  // (Steppers.cuh:489-561). Returns x in F[1] on both halves.
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    // b = rD (M CPT) + xu and r0 = b - A xu, meaningful on the primal half.
+    const double b = fma(rD, matvec(cpt), xu);
+    const double ax = fma(rD, matvec(x1sq * xu), xu);
+    double lo, up;
+    halves(-ax + b, &lo, &up);
+    double res = lo;  // r on the primal half, r~ = r on the shadow half
+    const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
+    const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
+    double dir = 0.0, x = xu, rho2 = 0.0;
+    BicgOut out{0, false, true};
+    for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
+      const double rho1 = dot_halves(res);  // r . r~
+      if (iter == 1) {
+        dir = res;
+        if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
+          out.rho_exit = true;
+          out.maxit = false;
+          break;
+        }
+      } else {
+        const double beta = rho1 / rho2;
+        dir = res + beta * dir;  // p (primal), p~ (shadow)
+      }
+      const double qv = fma(s2, matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
+      out.iterations++;
+      const double alpha = rho1 / dot_halves(primal_ ? qv : dir);  // p~ . q
+      x = x + alpha * dir;
+      res = res - alpha * qv;
+      rho2 = rho1;
+      if (tree32(bex::fabs_d(res), FmaxOp()) < a.BiCGResidualTolerance) {
+        out.maxit = false;
+        break;
+      }
+    }
+    halves(x, &lo, &up);
+    F[1] = lo;
+    return out;
+  }
+};
+
+// =========================================================================================
+// Time loop of one system (Solver.cuh:11-651), shared by the layouts.
+
+template <class L, int ALG>
+__device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   using D = BexSystemDims;
-  using G = Geometry<N>;
+  constexpr int N = L::kUnits;
   constexpr int UD = D::UD;
   constexpr int NUP = D::NUP;
   constexpr int NUA = D::NUA;
@@ -219,40 +427,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   constexpr int NE = D::NE;
   static_assert(UD == 2, "the implicit coupling acts on component 1 of a two-component unit");
 
-  extern __shared__ __align__(16) unsigned char smem[];
-  double* sM = reinterpret_cast<double*>(smem);
-  double* sMT = sM + N * N;
-  double2* sVec = reinterpret_cast<double2*>(sMT + N * N);
-  SysShared<D>* sSys = reinterpret_cast<SysShared<D>*>(sVec + G::SPB * N);
-  int* sStat = reinterpret_cast<int*>(sSys + G::SPB);
-
-  const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int wave = tid / kWave;
-  const int grp = lane / N;
-  const int uid = lane % N;
-  const int lsys = wave * G::SPW + grp;
-  const int sid = a.first_system + blockIdx.x * G::SPB + lsys;
-
-  // Coupling matrix and its transpose into LDS (the only workgroup barrier).
-  for (int k = tid; k < N * N; k += kBlock) {
-    const double v = a.CouplingMatrix[k];
-    sM[k] = v;
-    sMT[(k / N) + (k % N) * N] = v;
-  }
-  for (int k = tid; k < G::SPB * BEX_NSTAT; k += kBlock) sStat[k] = 0;
-  __syncthreads();
-
+  const int sid = a.first_system + lay.system_in_launch();
   const bool active = (sid < a.NS) && (sid < a.ActiveSystems);  // Solver.cuh:82,132
   if (!active) return;
 
-  SysShared<D>& S = sSys[lsys];
-  double2* vec = sVec + lsys * N;
-  int* st = sStat + lsys * BEX_NSTAT;
+  Sys& S = *lay.sys;
+  int* st = lay.stat;
+  const int uid = lay.uid();
+  const bool primal = lay.primal();
+  const bool leader = primal && uid == 0;
   const long long TLT = a.TLT;
   const long long gtid = (long long)(sid / a.ref_spb) * a.ref_block_stride +
                          (long long)(sid % a.ref_spb) * N + uid;  // Interface.cuh:875-879
   const int NS = a.NS;
+  if (leader && a.Timeline) a.Timeline[2 * (long long)sid] = (long long)wall_clock64();
 
   // System scope (Solver.cuh:84-129).
   if (uid == 0) {
@@ -311,16 +499,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                   a.IntegerGlobalParameters, uacc, iuacc,
                                                   S.sACC, S.isACC);
 
-  // Dense output state (DenseOutput.cuh:313-395); the index lives in S.DOIDX.
+  // Dense output (DenseOutput.cuh:313-395); the index lives in S.DOIDX.
   int updateDO = 0, skipped = 0;
   double doTime = 0.0;
   auto store_dense = [&](int updateStep) {
     if (updateDO == 1) {
       const int idx = S.DOIDX;
+      if (primal) {
 #pragma unroll
-      for (int i = 0; i < UD; i++)
-        a.DenseOutputStates[gtid + i * TLT + (long long)idx * TLT * UD] = x[i];
-      if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = t;
+        for (int i = 0; i < UD; i++)
+          a.DenseOutputStates[gtid + i * TLT + (long long)idx * TLT * UD] = x[i];
+        if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = t;
+      }
       wave_sync();
       if (uid == 0) S.DOIDX = idx + 1;
       wave_sync();
@@ -342,7 +532,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       dt = TD1 - t;
       endReached = 1;
     }
-    if (uid == 0) st[BEX_STAT_ATTEMPTS]++;
+    if (leader) st[BEX_STAT_ATTEMPTS]++;
 
     bool finiteLane = true;
     double k1[UD], k2[UD], k3[UD], k4[UD], k5[UD], k6[UD];
@@ -396,8 +586,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       } else {
         Ts = (s == 0) ? t : ((s == 3) ? t + dt : t + 0.5 * dt);
       }
-      const BicgOut o = stage_rhs<N>(F, xs, Ts, sid, uid, upar, S, uacc, iuacc, a, sM, sMT, vec);
-      if (uid == 0) {
+      // Right-hand side and coupling solve (Steppers.cuh:431-562).
+      double cpt, cpf;
+      CoupledSystems_PerBlock_OdeFunction<double>(sid, uid, F, xs, Ts, upar, S.sPAR,
+                                                  a.GlobalParameters,
+                                                  a.IntegerGlobalParameters, uacc, iuacc,
+                                                  S.sACC, S.isACC, &cpt, &cpf);
+      const BicgOut o = lay.coupling(F, xs[0], cpt, cpf, a);
+      if (leader) {
         st[BEX_STAT_STAGES]++;
         st[BEX_STAT_BICG_ITERATIONS] += o.iterations;
         st[BEX_STAT_BICG_RHO_EXITS] += o.rho_exit ? 1 : 0;
@@ -438,7 +634,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (!bex::is_finite(xn[i])) finiteLane = false;
       }
     }
-    int isFinite = group_all<N>(finiteLane, grp) ? 1 : 0;
+    int isFinite = lay.all(finiteLane) ? 1 : 0;
 
     if (ALG == kRKCK45) {
       // Error controller (ErrorControllers.cuh:453-542).
@@ -451,8 +647,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         ru = ru && (err[i] < tol);
         rr = bex::fmin_d(rr, tol / err[i]);
       }
-      updateStep = group_all<N>(ru, grp) ? 1 : 0;
-      const double relErr = group_min_keep<N>(rr);
+      updateStep = lay.all(ru) ? 1 : 0;
+      const double relErr = lay.min_keep(rr);
       double mult = (updateStep == 1) ? 0.8 * bex::pow(relErr, 1.0 / 5.0)
                                       : 0.8 * bex::pow(relErr, 1.0 / 4.0);
       if (!bex::is_finite(mult)) isFinite = 0;
@@ -461,11 +657,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         updateStep = 0;
         if (dt < (a.MinimumTimeStep * 1.01)) {
           terminate = 1;
-          if (uid == 0) st[BEX_STAT_TERMINATED]++;
+          if (leader) st[BEX_STAT_TERMINATED]++;
         }
       } else if (dt < (a.MinimumTimeStep * 1.01)) {
         updateStep = 1;
-        if (uid == 0) st[BEX_STAT_FORCED_ACCEPTS]++;
+        if (leader) st[BEX_STAT_FORCED_ACCEPTS]++;
       }
       mult = bex::fmin_d(mult, a.TimeStepGrowLimit);
       mult = bex::fmax_d(mult, a.TimeStepShrinkLimit);
@@ -477,7 +673,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       if (isFinite == 0) {
         terminate = 1;
         updateStep = 0;
-        if (uid == 0) st[BEX_STAT_TERMINATED]++;
+        if (leader) st[BEX_STAT_TERMINATED]++;
       }
       dtNew = a.InitialTimeStep;
     }
@@ -501,10 +697,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
           }
         }
       }
-      if (group_any<N>(corrLane, grp)) {
-        const double evStep = group_min_keep<N>(evLane);
+      if (lay.any(corrLane)) {
+        const double evStep = lay.min_keep(evLane);
         if (evStep < a.MinimumTimeStep) {
-          if (uid == 0) st[BEX_STAT_EVENTS_OMITTED]++;
+          if (leader) st[BEX_STAT_EVENTS_OMITTED]++;
         } else {
           dtNew = evStep;
           updateStep = 0;
@@ -517,7 +713,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
       for (int i = 0; i < UD; i++) x[i] = xn[i];
       if (uid == 0) {
-        st[BEX_STAT_ACCEPTED]++;
+        if (leader) st[BEX_STAT_ACCEPTED]++;
         S.T = t;
         S.dT = dt;
       }
@@ -534,7 +730,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             CoupledSystems_PerBlock_ActionAfterEventDetection<double>(
                 sid, uid, i, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
                 a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
-            atomicAdd(&st[BEX_STAT_EVENTS], 1);
+            if (primal) atomicAdd(&st[BEX_STAT_EVENTS], 1);
           }
         }
         wave_sync();
@@ -547,7 +743,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       }
       wave_sync();
       t = S.T;
-    } else if (uid == 0) {
+    } else if (leader) {
       st[BEX_STAT_REJECTED]++;
     }
     const int udt = S.UDT;
@@ -576,17 +772,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                a.IntegerGlobalParameters, uacc, iuacc, S.sACC,
                                                S.isACC);
   wave_sync();
-  // Recompute the write-back addresses here instead of keeping the load-time ones alive
-  // across the time loop (they would occupy ~2 registers per accessory).
-  long long wb = gtid;
-  asm volatile("" : "+v"(wb));
+  if (primal) {
+    // Recompute the write-back addresses here instead of keeping the load-time ones alive
+    // across the time loop (they would occupy ~2 registers per accessory).
+    long long wb = gtid;
+    asm volatile("" : "+v"(wb));
 #pragma unroll
-  for (int i = 0; i < UD; i++) a.ActualState[wb + i * TLT] = x[i];
+    for (int i = 0; i < UD; i++) a.ActualState[wb + i * TLT] = x[i];
 #pragma nounroll
-  for (int i = 0; i < NUA; i++) a.UnitAccessories[wb + i * TLT] = uacc[i];
+    for (int i = 0; i < NUA; i++) a.UnitAccessories[wb + i * TLT] = uacc[i];
 #pragma unroll
-  for (int i = 0; i < NiUA; i++) a.IntegerUnitAccessories[wb + i * TLT] = iuacc[i];
-  if (uid == 0) {
+    for (int i = 0; i < NiUA; i++) a.IntegerUnitAccessories[wb + i * TLT] = iuacc[i];
+  }
+  if (leader) {
     a.ActualTime[sid] = S.T;
     a.TimeDomain[sid] = S.TD[0];
     a.TimeDomain[sid + NS] = S.TD[1];
@@ -594,7 +792,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     for (int i = 0; i < D::NiSA; i++) a.IntegerSystemAccessories[sid + i * NS] = S.isACC[i];
     if (a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
   }
-  for (int k = uid; k < BEX_NSTAT; k += N) a.Stats[(long long)sid * BEX_NSTAT + k] += st[k];
+  wave_sync();
+  if (primal)
+    for (int k = uid; k < BEX_NSTAT; k += N) a.Stats[(long long)sid * BEX_NSTAT + k] += st[k];
+  if (leader && a.Timeline) a.Timeline[2 * (long long)sid + 1] = (long long)wall_clock64();
+}
+
+template <class L, int ALG>
+__global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
+void solve_kernel(const BexKernelArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  L lay;
+  lay.init(smem, a);
+  integrate<L, ALG>(lay, a);
 }
 
 }  // namespace bexk

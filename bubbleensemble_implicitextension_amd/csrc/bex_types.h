@@ -42,7 +42,8 @@ struct BexKernelArgs {
   int* IntegerUnitAccessories;
   double* SystemAccessories;
   int* IntegerSystemAccessories;
-  const double* CouplingMatrix;  // coupling 0, column-major [row + col*N]
+  const double* CouplingMatrix;   // coupling 0, column-major [row + col*N]
+  const double* CouplingMatrixT;  // its transpose, column-major
   int* DenseOutputIndex;
   double* DenseOutputTimeInstances;
   double* DenseOutputStates;
@@ -51,6 +52,7 @@ struct BexKernelArgs {
   const double* EventTolerance;
   const int* EventDirection;
   long long* Stats;  // [NS][BEX_NSTAT], accumulated
+  long long* Timeline;  // optional [NS][2]: device wall-clock ticks at system start and end
 
   double InitialTimeStep;
   double MaximumTimeStep;

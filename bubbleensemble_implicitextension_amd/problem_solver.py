@@ -61,6 +61,10 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     BiCGRhoTolerance = 14
     BiCGResidualTolerance = 15
     BiCGMaxIterations = 16
+    KernelLayout = 17
+
+
+LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR = 0, 1, 2
 
 
 # Reference-style bare names (the reference uses unscoped enums).
@@ -214,6 +218,22 @@ class ProblemSolver:
         ms = ctypes.c_double()
         self._check(self._lib.bex_timer_stop(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def enable_timeline(self, on=True):
+        self._check(self._lib.bex_enable_timeline(self._h, 1 if on else 0))
+
+    def timeline(self):
+        """[NS, 2] start/end seconds of each system in the last launch (relative to the
+        earliest start)."""
+        out = np.zeros(2 * self.NS, dtype=np.int64)
+        tps = ctypes.c_double()
+        self._check(self._lib.bex_get_timeline(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), ctypes.byref(tps)))
+        t = out.reshape(self.NS, 2).astype(np.float64)
+        valid = t[:, 0] > 0
+        if valid.any():
+            t -= t[valid, 0].min()
+        return t / tps.value
 
     def reset_stats(self):
         self._check(self._lib.bex_reset_stats(self._h))

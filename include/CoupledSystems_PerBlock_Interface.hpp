@@ -63,7 +63,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   SharedCouplingMatrices,
   BiCGRhoTolerance,
   BiCGResidualTolerance,
-  BiCGMaxIterations
+  BiCGMaxIterations,
+  KernelLayout
 };
 
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,

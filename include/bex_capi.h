@@ -78,8 +78,14 @@ enum bex_option {
   /* additions: the reference hard-codes these BiCG constants (Steppers.cuh:508,522,558) */
   BEX_BiCGRhoTolerance,
   BEX_BiCGResidualTolerance,
-  BEX_BiCGMaxIterations
+  BEX_BiCGMaxIterations,
+  /* lane layout of the solver kernel: BEX_LAYOUT_* (no reference counterpart) */
+  BEX_KernelLayout
 };
+
+#define BEX_LAYOUT_AUTO 0      /* role pairs for 32 units, lane groups otherwise */
+#define BEX_LAYOUT_GROUP 1     /* lane = unit, 64/N systems per wave, M in LDS */
+#define BEX_LAYOUT_ROLE_PAIR 2 /* 32 units: one system per wave, M / M^T rows in registers */
 
 /* The reference's template parameters (Interface.cuh:135). */
 typedef struct {
@@ -148,6 +154,12 @@ int bex_reset_stats(bex_solver* s);
  * counterpart (the reference times its drivers with clock()). */
 int bex_timer_start(bex_solver* s);
 int bex_timer_stop(bex_solver* s, double* elapsed_ms);
+
+/* Per-system timeline of the most recent launch: when enabled, the kernel stores the device
+ * wall-clock counter at the start and end of each system ([NS][2]); bex_get_timeline copies
+ * it and returns the counter frequency. No reference counterpart. */
+int bex_enable_timeline(bex_solver* s, int on);
+int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second);
 
 /* Message of the last failing call on this handle (or of the last bex_create failure when
  * s is NULL). */

@@ -2,12 +2,15 @@
 import numpy as np
 
 from bubbleensemble_implicitextension_amd import collapse
+from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
 from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
 
-def run_gpu(ens, periods, algorithm=1, options=None, ndo=0, reset_after=None, end_time=1.0):
+def run_gpu(ens, periods, algorithm=1, options=None, ndo=0, reset_after=None, end_time=1.0,
+            layout=0):
     s = collapse.make_solver(ens, algorithm=algorithm, NDO=ndo)
     collapse.fill(s, ens)
+    s.SolverOption(O.KernelLayout, layout)
     s.host_view(V.TimeDomain)[ens.NS:] = end_time
     if options:
         collapse.apply_options(s, options)

@@ -1,33 +1,39 @@
-"""GPU parity: the gfx950 kernel against the CPU oracle's canonical mode, bit for bit.
+"""GPU parity: the gfx950 kernels against the CPU oracle's canonical mode, bit for bit.
 
 The canonical oracle restates the reference solver path with the same elementary functions
-and association order as the kernel, so every state value, accessory, event counter and
-step/iteration counter must agree exactly, including on chaotic systems.
+and association order as the kernels, so every state value, accessory, event counter and
+step/iteration counter must agree exactly, including on chaotic systems. Both lane layouts of
+the 32-unit kernel (role pairs with DPP broadcasts, lane groups with LDS) are checked.
 """
 import numpy as np
 import pytest
 
 from bubbleensemble_implicitextension_amd import workloads as wl
+from bubbleensemble_implicitextension_amd.problem_solver import LAYOUT_GROUP, LAYOUT_ROLE_PAIR
 
 from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
 
 pytestmark = pytest.mark.gpu
 
+CASES = [(8, LAYOUT_GROUP), (16, LAYOUT_GROUP), (32, LAYOUT_GROUP), (32, LAYOUT_ROLE_PAIR),
+         (64, LAYOUT_GROUP)]
 
-@pytest.mark.parametrize("N", [8, 16, 32, 64])
-def test_small_ensembles_bitexact(oracle, N):
+
+@pytest.mark.parametrize("N,layout", CASES)
+def test_small_ensembles_bitexact(oracle, N, layout):
     ens = wl.small_ensemble(n=N, ns=9, seed=11 + N)
-    print(f"small ensemble N={N} NS={ens.NS}", flush=True)
-    s = run_gpu(ens, periods=1)
+    print(f"small ensemble N={N} NS={ens.NS} layout={layout}", flush=True)
+    s = run_gpu(ens, periods=1, layout=layout)
     p = run_oracle(oracle, ens, periods=1)
     compare_solver_to_oracle(s, p)
     s.close()
 
 
-def test_rk4_bitexact(oracle):
-    ens = wl.small_ensemble(n=16, ns=4, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))
+@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
+def test_rk4_bitexact(oracle, layout):
+    ens = wl.small_ensemble(n=32, ns=4, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))
     opts = {"InitialTimeStep": 2e-4}
-    s = run_gpu(ens, periods=1, algorithm=0, options=opts)
+    s = run_gpu(ens, periods=1, algorithm=0, options=opts, layout=layout)
     p = run_oracle(oracle, ens, periods=1, algorithm=0, options=opts)
     compare_solver_to_oracle(s, p)
 
@@ -38,14 +44,23 @@ def config0_subset_sids():
     return [s for s in range(64) if s != 7]
 
 
-def test_config0_two_periods_with_reset(oracle):
+@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
+def test_config0_two_periods_with_reset(oracle, layout):
     ens = wl.config0().subset(config0_subset_sids())
-    print(f"config0 subset NS={ens.NS}", flush=True)
-    s = run_gpu(ens, periods=2, reset_after=1)
+    print(f"config0 subset NS={ens.NS} layout={layout}", flush=True)
+    s = run_gpu(ens, periods=2, reset_after=1, layout=layout)
     p = run_oracle(oracle, ens, periods=2, reset_after=1)
     compare_solver_to_oracle(s, p)
     st = s.stats()
     assert st[:, 6].sum() > 0, "no collapse events were recorded"
+
+
+def test_singular_corner_short_window(oracle):
+    """The Mach-1 corner system over the first 5 % of a period (golden case cfg0_corner)."""
+    ens = wl.config0().subset([7])
+    s = run_gpu(ens, periods=1, end_time=0.05)
+    p = run_oracle(oracle, ens, periods=1, end_time=0.05)
+    compare_solver_to_oracle(s, p)
 
 
 def test_config0_within_config1_batch(oracle):
@@ -55,7 +70,6 @@ def test_config0_within_config1_batch(oracle):
     big = wl.config1()
     sids = wl.config0_in_config1_sids()
     keep = [k for k, sid in enumerate(sids) if k != 7]
-    big.options = dict(big.options)
     s = run_gpu(big, periods=1, end_time=0.2)
     small = wl.config0().subset(keep)
     p = run_oracle(oracle, small, periods=1, end_time=0.2)
