@@ -64,8 +64,8 @@ def cpu_baseline(seconds_budget=20.0, sample=48, seed=7):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--end-time", type=float, default=1.0,
                     help="fraction of a period per step (profiling only)")

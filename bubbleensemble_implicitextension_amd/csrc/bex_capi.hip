@@ -58,6 +58,7 @@ struct bex_solver {
   double BiCGRhoTolerance = 1e-6, BiCGResidualTolerance = 1e-6;
   int BiCGMaxIterations = 19;
   int KernelLayout = BEX_LAYOUT_AUTO;
+  int PriorityStepThreshold = 3000;
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
   std::string err;
 };
@@ -436,6 +437,7 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_BiCGRhoTolerance: s->BiCGRhoTolerance = v; break;
     case BEX_BiCGResidualTolerance: s->BiCGResidualTolerance = v; break;
     case BEX_BiCGMaxIterations: s->BiCGMaxIterations = (int)v; break;
+    case BEX_PriorityStepThreshold: s->PriorityStepThreshold = (int)v; break;
     case BEX_KernelLayout: {
       KernelChoice kc;
       if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, &kc))
@@ -779,6 +781,7 @@ int bex_solve(bex_solver* s) {
   a.BiCGRhoTolerance = s->BiCGRhoTolerance;
   a.BiCGResidualTolerance = s->BiCGResidualTolerance;
   a.BiCGMaxIterations = s->BiCGMaxIterations;
+  a.PriorityStepThreshold = s->PriorityStepThreshold > 0 ? s->PriorityStepThreshold : 0x3fffffff;
   a.ref_spb = s->d.SPB;
   a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;

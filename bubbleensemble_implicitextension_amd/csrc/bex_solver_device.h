@@ -17,9 +17,9 @@
 //                     exchanged through per-system LDS slots.
 //
 // Arithmetic order is fixed (explicit fma, -ffp-contract=off) and identical to the CPU oracle's
-// canonical mode: sequential fma accumulation over columns for every matrix-vector product,
-// the reference's halving-tree order for every dot product. The kernels reproduce the oracle
-// bit for bit.
+// canonical mode: every matrix-vector row is four fma chains over the columns j = q (mod 4)
+// combined as (s0 + s1) + (s2 + s3), every dot product follows the reference's halving tree.
+// The kernels reproduce the oracle bit for bit.
 //
 // Reference: Solver.cuh = SourceCodes/CoupledSystmes_PerBlock_Solver.cuh,
 // Steppers.cuh = SourceCodes/CoupledSystems_PerBlock_ExplicitRungeKutta_Steppers.cuh,
@@ -166,6 +166,21 @@ struct GroupLayout {
     return v;
   }
 
+  // Row product in the canonical order: four partial sums over columns j = q (mod 4), each a
+  // sequential fma chain, combined as (s0 + s1) + (s2 + s3).
+  template <bool Y>
+  __device__ __forceinline__ double row_dot4(const double* mat) const {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < N; j += 4) {
+      s0 = fma(mat[uid_ + j * N], Y ? vec[j].y : vec[j].x, s0);
+      s1 = fma(mat[uid_ + (j + 1) * N], Y ? vec[j + 1].y : vec[j + 1].x, s1);
+      s2 = fma(mat[uid_ + (j + 2) * N], Y ? vec[j + 2].y : vec[j + 2].x, s2);
+      s3 = fma(mat[uid_ + (j + 3) * N], Y ? vec[j + 3].y : vec[j + 3].x, s3);
+    }
+    return (s0 + s1) + (s2 + s3);
+  }
+
   // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
   // (Steppers.cuh:489-561). Returns x in F[1].
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
@@ -175,23 +190,17 @@ struct GroupLayout {
     wave_sync();
     vec[uid_].x = cpt;
     wave_sync();
-    double acc = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < N; j++) acc = fma(sM[uid_ + j * N], vec[j].x, acc);
-    const double b = fma(rD, acc, xu);
+    const double b = fma(rD, row_dot4<false>(sM), xu);
     wave_sync();
     vec[uid_].x = x1sq * xu;
     wave_sync();
-    acc = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < N; j++) acc = fma(sM[uid_ + j * N], vec[j].x, acc);
-    const double ax = fma(rD, acc, xu);
+    const double ax = fma(rD, row_dot4<false>(sM), xu);
     double r = -ax + b;
     double rt = r;
     double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
+    double rho1 = sum(r * rt);
     BicgOut out{0, false, true};
     for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
-      const double rho1 = sum(r * rt);
       if (iter == 1) {
         p = r;
         pt = rt;
@@ -208,13 +217,8 @@ struct GroupLayout {
       wave_sync();
       vec[uid_] = make_double2(x1sq * p, rD * pt);
       wave_sync();
-      double a1 = 0.0, a2 = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < N; j++) {
-        const double2 v = vec[j];
-        a1 = fma(sM[uid_ + j * N], v.x, a1);
-        a2 = fma(sMT[uid_ + j * N], v.y, a2);
-      }
+      const double a1 = row_dot4<false>(sM);
+      const double a2 = row_dot4<true>(sMT);
       const double q = fma(rD, a1, p);
       const double qt = fma(x1sq, a2, pt);
       out.iterations++;
@@ -223,7 +227,10 @@ struct GroupLayout {
       r = r - alpha * q;
       rt = rt - alpha * qt;
       rho2 = rho1;
-      if (fmax_all(bex::fabs_d(r)) < a.BiCGResidualTolerance) {
+      // The residual test and the next iteration's r.r~ are independent: evaluate together.
+      const double rmax = fmax_all(bex::fabs_d(r));
+      rho1 = sum(r * rt);
+      if (rmax < a.BiCGResidualTolerance) {
         out.maxit = false;
         break;
       }
@@ -236,28 +243,29 @@ struct GroupLayout {
 // =========================================================================================
 // RolePairLayout32: one N = 32 system per wave, primal and shadow halves.
 
-// acc = fma(z(lane k of the lane's 16-lane row), m[k], acc) for k = 0..15, in order. The
-// s_nop covers the DPP read-after-VALU-write hazard of z.
-__device__ __forceinline__ void fmac_row16(double& acc, double z, const double* m) {
+// acc[j % 4] = fma(z(lane j of the lane's 16-lane row), m[j], acc[j % 4]) for j = 0..15, in
+// order: the canonical four-partial-sum order of every matrix-vector product. The s_nop covers
+// the DPP read-after-VALU-write hazard of z.
+__device__ __forceinline__ void fmac_row16(double* acc, double z, const double* m) {
   asm volatile(
       "s_nop 1\n"
-      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %14 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %15 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %16 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %1, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
-      : "+v"(acc)
+      "v_fmac_f64_dpp %0, %4, %5 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %4, %6 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %4, %7 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %4, %8 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %4, %9 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %4, %10 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %4, %11 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %4, %12 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %4, %13 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %4, %14 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %4, %15 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %4, %16 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %4, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %4, %18 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %4, %19 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %4, %20 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])
       : "v"(z), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]),
         "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]),
         "v"(m[14]), "v"(m[15]));
@@ -350,14 +358,15 @@ struct RolePairLayout32 {
     return uniform(v);
   }
 
-  // acc_i = sum_j mrow[j] * z_j over j = 0..31 (sequential fma), z from the lane's half.
+  // acc_i = sum_j mrow[j] * z_j over j = 0..31 in the canonical order (four partial sums),
+  // z taken from the lane's half.
   __device__ __forceinline__ double matvec(double z) const {
     double even, odd;
     rows_of_half(z, &even, &odd);
-    double acc = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     fmac_row16(acc, even, &mrow[0]);
     fmac_row16(acc, odd, &mrow[16]);
-    return acc;
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
   }
 
   // sum_i a_i b_i (tree order) for a on the primal half and b on the shadow half of v.
@@ -382,9 +391,9 @@ struct RolePairLayout32 {
     const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
     const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
     double dir = 0.0, x = xu, rho2 = 0.0;
+    double rho1 = dot_halves(res);  // r . r~
     BicgOut out{0, false, true};
     for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
-      const double rho1 = dot_halves(res);  // r . r~
       if (iter == 1) {
         dir = res;
         if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
@@ -402,7 +411,10 @@ struct RolePairLayout32 {
       x = x + alpha * dir;
       res = res - alpha * qv;
       rho2 = rho1;
-      if (tree32(bex::fabs_d(res), FmaxOp()) < a.BiCGResidualTolerance) {
+      // The residual test and the next iteration's r.r~ are independent: evaluate together.
+      const double rmax = tree32(bex::fabs_d(res), FmaxOp());
+      rho1 = dot_halves(res);
+      if (rmax < a.BiCGResidualTolerance) {
         out.maxit = false;
         break;
       }
@@ -525,6 +537,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   }
 
   int terminate = 0;
+  int accepted = 0;
   while (terminate == 0) {  // Solver.cuh:329-572
     int updateStep = 1, endReached = 0;
     dt = dtNew;
@@ -712,6 +725,13 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       t += dt;
 #pragma unroll
       for (int i = 0; i < UD; i++) x[i] = xn[i];
+      // Systems that keep stepping long after the typical one become the launch's tail: raise
+      // the issue priority of their wave step by step so they run at close to single-wave
+      // latency while the remaining waves fill the idle issue slots.
+      ++accepted;
+      if (accepted == a.PriorityStepThreshold) __builtin_amdgcn_s_setprio(1);
+      if (accepted == 2 * a.PriorityStepThreshold) __builtin_amdgcn_s_setprio(2);
+      if (accepted == 4 * a.PriorityStepThreshold) __builtin_amdgcn_s_setprio(3);
       if (uid == 0) {
         if (leader) st[BEX_STAT_ACCEPTED]++;
         S.T = t;

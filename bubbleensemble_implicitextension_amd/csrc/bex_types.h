@@ -64,4 +64,5 @@ struct BexKernelArgs {
   double BiCGRhoTolerance;
   double BiCGResidualTolerance;
   int BiCGMaxIterations;
+  int PriorityStepThreshold;  // accepted steps after which a system's wave priority rises
 };

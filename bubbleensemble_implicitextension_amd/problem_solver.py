@@ -62,6 +62,7 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     BiCGResidualTolerance = 15
     BiCGMaxIterations = 16
     KernelLayout = 17
+    PriorityStepThreshold = 18
 
 
 LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR = 0, 1, 2

@@ -64,7 +64,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   BiCGRhoTolerance,
   BiCGResidualTolerance,
   BiCGMaxIterations,
-  KernelLayout
+  KernelLayout,
+  PriorityStepThreshold
 };
 
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,

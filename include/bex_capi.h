@@ -80,7 +80,10 @@ enum bex_option {
   BEX_BiCGResidualTolerance,
   BEX_BiCGMaxIterations,
   /* lane layout of the solver kernel: BEX_LAYOUT_* (no reference counterpart) */
-  BEX_KernelLayout
+  BEX_KernelLayout,
+  /* accepted steps (per launch) after which a system's wave priority rises; 0 disables
+   * (no reference counterpart; default 3000) */
+  BEX_PriorityStepThreshold
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32 units, lane groups otherwise */

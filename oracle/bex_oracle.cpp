@@ -157,6 +157,18 @@ struct BicgWork {
   }
 };
 
+// Canonical row product sum_j A_ij v_j (A = M, or M^T when transposed; M column-major): four
+// partial sums over the columns j = q (mod 4), each a sequential fma chain in increasing j,
+// combined as (s0 + s1) + (s2 + s3). The GPU kernels accumulate in exactly this order.
+inline double row_dot4(const double* M, int N, int i, bool transposed, const double* v) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < N; j++) {
+    const double a = transposed ? M[j + i * N] : M[i + j * N];
+    s[j & 3] = std::fma(a, v[j], s[j & 3]);
+  }
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 struct BicgResult {
   int iterations;  // matrix-vector pairs evaluated
   bool rho_exit;   // left on the |rho_1| test of the first iteration
@@ -177,7 +189,7 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
       for (int j = 0; j < N; j++) acc += M[i + j * N] * CPT[j];
       w.b[i] = rD[i] * acc + x[i];
     } else {
-      for (int j = 0; j < N; j++) acc = std::fma(M[i + j * N], CPT[j], acc);
+      acc = row_dot4(M, N, i, false, CPT);
       w.b[i] = std::fma(rD[i], acc, x[i]);
     }
   }
@@ -190,7 +202,7 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
       for (int j = 0; j < N; j++) acc += (M[i + j * N] * w.x1sq[j]) * x[j];
       ax = x[i] + rD[i] * acc;
     } else {
-      for (int j = 0; j < N; j++) acc = std::fma(M[i + j * N], w.u[j], acc);
+      acc = row_dot4(M, N, i, false, w.u.data());
       ax = std::fma(rD[i], acc, x[i]);
     }
     w.r[i] = -ax + w.b[i];
@@ -234,11 +246,8 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
         w.w[j] = rD[j] * w.pt[j];
       }
       for (int i = 0; i < N; i++) {
-        double a1 = 0.0, a2 = 0.0;
-        for (int j = 0; j < N; j++) {
-          a1 = std::fma(M[i + j * N], w.u[j], a1);
-          a2 = std::fma(M[j + i * N], w.w[j], a2);
-        }
+        const double a1 = row_dot4(M, N, i, false, w.u.data());
+        const double a2 = row_dot4(M, N, i, true, w.w.data());
         w.q[i] = std::fma(rD[i], a1, w.p[i]);
         w.qt[i] = std::fma(w.x1sq[i], a2, w.pt[i]);
       }
