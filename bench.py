@@ -37,10 +37,10 @@ def flops_fm1(stats, n):
     return stages * (4 * n * n + 47 * n) + iters * (4 * n * n + 21 * n) + attempts * 140 * n
 
 
-def cpu_baseline(seconds_budget=20.0, sample=48, seed=7):
-    """Host restatement (literal mode, glibc, reference association), OpenMP over systems on
-    the host cores, one period from rest for a seeded random sample of the workload's
-    systems."""
+def cpu_baseline(sample=2048, seed=7):
+    """Host restatement (literal mode: glibc, reference association), OpenMP over systems on
+    the host cores (dynamic schedule), one period from rest for a seeded random sample of the
+    workload's systems (about 15 s with 16 threads)."""
     from bubbleensemble_implicitextension_amd import workloads as wl
     from oracle import pyoracle
 
@@ -56,9 +56,37 @@ def cpu_baseline(seconds_budget=20.0, sample=48, seed=7):
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "system-periods/s", "cores": threads,
             "kind": "port",
-            "sample": f"{sample} seeded-random systems of the same 16384-system workload, "
-                      f"first period from rest, literal host restatement, {threads} OpenMP "
-                      f"threads, {dt:.1f} s"}
+            "sample": f"{sample} seeded-random systems (seed {seed}) of the same 16384-system "
+                      f"workload, first period from rest, literal host restatement, {threads} "
+                      f"OpenMP threads, {dt:.1f} s"}
+
+
+def measured_traffic(kernel_tag):
+    """HBM bytes per launch of the solver kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading for a plain
+    device copy of known size). None when the summary is absent."""
+    path = os.path.join(REPO, "profiles", "solve_kernel_n32_16384sys_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch[" + kernel_tag + "]:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration():
+    """Plain 1 GiB device-to-device copy (2^27 doubles), for scaling the FETCH_SIZE and
+    WRITE_SIZE counters: run under rocprofv3 --pmc and compare with 1 GiB."""
+    import torch
+
+    n = 1 << 27
+    x = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8}), flush=True)
+    return 0
 
 
 def main():
@@ -69,7 +97,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--end-time", type=float, default=1.0,
                     help="fraction of a period per step (profiling only)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(systems are independent: shard them across GPUs by cost, see DESIGN.md)",
@@ -128,8 +160,9 @@ def main():
         "bicg_iterations_per_rhs": iters_per_rhs,
         "steps_accepted_per_system_period": float(stats[:, 0].sum()) / (ens.NS * args.steps),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "bexk::solve_kernel<32, RKCK45>",
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("period") if args.end_time == 1.0 else None,
+                     "kernel": "bexk::solve_kernel<RolePairLayout32, RKCK45>",
                      "kernel_ms_per_launch": kernel_ms / args.steps,
                      "flops_per_launch": flops / args.steps},
     }
