@@ -46,17 +46,10 @@ BEX_HD double quiet_nan() { return from_bits(0x7ff8000000000000ULL); }
 BEX_HD double infinity() { return from_bits(0x7ff0000000000000ULL); }
 
 // fmin/fmax with the C99 NaN rule (a NaN operand is ignored), written out so that the
-// host and device agree on every input.
-BEX_HD double fmin_d(double a, double b) {
-  if (is_nan(a)) return b;
-  if (is_nan(b)) return a;
-  return (b < a) ? b : a;
-}
-BEX_HD double fmax_d(double a, double b) {
-  if (is_nan(a)) return b;
-  if (is_nan(b)) return a;
-  return (b > a) ? b : a;
-}
+// host and device agree on every input: b is taken when it is smaller (larger) or when a is
+// NaN, otherwise a (which covers b NaN). One select, no branches.
+BEX_HD double fmin_d(double a, double b) { return ((b < a) | is_nan(a)) ? b : a; }
+BEX_HD double fmax_d(double a, double b) { return ((b > a) | is_nan(a)) ? b : a; }
 BEX_HD double fabs_d(double x) { return from_bits(as_bits(x) & 0x7fffffffffffffffULL); }
 
 // ---------------------------------------------------------------------------------------

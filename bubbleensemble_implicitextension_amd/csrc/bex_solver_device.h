@@ -304,6 +304,27 @@ __device__ __forceinline__ double tree32(double s, Op op) {
   return uniform(s);
 }
 
+// Two independent halving trees evaluated in lockstep, so that each one's cross-lane moves
+// fill the other's dependency stalls; same per-tree order as tree32.
+template <class OpA, class OpB>
+__device__ __forceinline__ void tree32x2(double& a, double& b, OpA opa, OpB opb) {
+  double ea, oa, eb, ob;
+  rows_of_half(a, &ea, &oa);
+  rows_of_half(b, &eb, &ob);
+  a = opa(a, oa);
+  b = opb(b, ob);
+  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x108, 0xf, 0xf, true));
+  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x108, 0xf, 0xf, true));
+  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x104, 0xf, 0xf, true));
+  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x104, 0xf, 0xf, true));
+  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x102, 0xf, 0xf, true));
+  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x102, 0xf, 0xf, true));
+  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x101, 0xf, 0xf, true));
+  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x101, 0xf, 0xf, true));
+  a = uniform(a);
+  b = uniform(b);
+}
+
 struct AddOp {
   __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
 };
@@ -412,8 +433,11 @@ struct RolePairLayout32 {
       res = res - alpha * qv;
       rho2 = rho1;
       // The residual test and the next iteration's r.r~ are independent: evaluate together.
-      const double rmax = tree32(bex::fabs_d(res), FmaxOp());
-      rho1 = dot_halves(res);
+      double lo2, up2;
+      halves(res, &lo2, &up2);
+      double rmax = bex::fabs_d(res);
+      rho1 = lo2 * up2;
+      tree32x2(rmax, rho1, FmaxOp(), AddOp());
       if (rmax < a.BiCGResidualTolerance) {
         out.maxit = false;
         break;
