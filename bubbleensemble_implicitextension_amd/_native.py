@@ -70,6 +70,22 @@ _SIGNATURES = {
     "bex_enable_timeline": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "bex_get_timeline": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_set_warm_start_network": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "bex_enable_sampling": ([ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int], ctypes.c_int),
+    "bex_get_samples": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_longlong,
+                         ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "bex_mlp_workspace_floats": ([ctypes.c_longlong], ctypes.c_longlong),
+    "bex_mlp_loss_grad": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                          ctypes.c_int),
+    "bex_mlp_adam": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float, ctypes.c_void_p], ctypes.c_int),
+    "bex_mlp_predict": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p,
+                         ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "bex_mlp_predict_mfma": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
+                              ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p],
+                             ctypes.c_int),
     "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
 }
 

@@ -35,7 +35,7 @@ COMMON_FLAGS = [
 
 
 def build_library(verbose=False, extra_flags=(), output=LIB):
-    sources = [os.path.join(CSRC, "bex_capi.hip")]
+    sources = [os.path.join(CSRC, "bex_capi.hip"), os.path.join(CSRC, "bex_mlp.hip")]
     cmd = [HIPCC, *COMMON_FLAGS, *extra_flags, "-shared", "-o", output + ".tmp", *sources]
     if verbose:
         cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")

@@ -6,6 +6,7 @@
 // and launches the solver kernel of bex_solver_device.h once per Solve().
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -59,6 +60,12 @@ struct bex_solver {
   int BiCGMaxIterations = 19;
   int KernelLayout = BEX_LAYOUT_AUTO;
   int PriorityStepThreshold = 3000;
+  int WarmStart = 0;                          // 1: learned BiCG initial guess
+  BexMlpDevice* d_net = nullptr;              // network in the kernel's operand layout
+  float* d_samples = nullptr;                 // [SampleCapacity][BEX_SAMPLE_FLOATS] samples
+  unsigned long long* d_sample_count = nullptr;
+  long long SampleCapacity = 0;
+  int SampleEvery = 0;
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
   std::string err;
 };
@@ -110,13 +117,18 @@ KernelChoice choice(int alg) {
 }
 
 // layout: BEX_LAYOUT_AUTO picks the role-pair layout for 32 units and the group layout
-// otherwise.
-bool kernel_for(int ups, int alg, int layout, KernelChoice* k) {
+// otherwise. mode: bexk::WarmStartMode; the learned guess and sample collection exist for the
+// role-pair layout only.
+bool kernel_for(int ups, int alg, int layout, int mode, KernelChoice* k) {
   if (ups == 32 && layout != BEX_LAYOUT_GROUP) {
-    *k = choice<bexk::RolePairLayout32>(alg);
-    return true;
+    switch (mode) {
+      case bexk::kPlain: *k = choice<bexk::RolePairLayout32<bexk::kPlain>>(alg); return true;
+      case bexk::kSampling: *k = choice<bexk::RolePairLayout32<bexk::kSampling>>(alg); return true;
+      case bexk::kLearned: *k = choice<bexk::RolePairLayout32<bexk::kLearned>>(alg); return true;
+      default: return false;
+    }
   }
-  if (layout == BEX_LAYOUT_ROLE_PAIR) return false;
+  if (layout == BEX_LAYOUT_ROLE_PAIR || mode != bexk::kPlain) return false;
   switch (ups) {
     case 8: *k = choice<bexk::GroupLayout<8>>(alg); return true;
     case 16: *k = choice<bexk::GroupLayout<16>>(alg); return true;
@@ -124,6 +136,25 @@ bool kernel_for(int ups, int alg, int layout, KernelChoice* k) {
     case 64: *k = choice<bexk::GroupLayout<64>>(alg); return true;
     default: return false;
   }
+}
+
+// Batch inference through the solver's own MFMA path: each wave evaluates 32 rows, row r on
+// lanes r % 32 and 32 + r % 32 exactly as a role-pair wave evaluates its 32 units.
+__global__ __launch_bounds__(64) void mlp_mfma_kernel(const float* __restrict__ rows, int stride,
+                                                      long long n, const BexMlpDevice* net,
+                                                      float* __restrict__ out) {
+  __shared__ BexMlpDevice snet;
+  const uint4* from = reinterpret_cast<const uint4*>(net);
+  uint4* to = reinterpret_cast<uint4*>(&snet);
+  for (int k = threadIdx.x; k < (int)(sizeof(BexMlpDevice) / sizeof(uint4)); k += 64) to[k] = from[k];
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const long long r = (long long)blockIdx.x * 32 + (lane & 31);
+  double wf[BEX_MLP_IN];
+#pragma unroll
+  for (int k = 0; k < BEX_MLP_IN; k++) wf[k] = (r < n) ? (double)rows[r * stride + k] : 0.0;
+  const float y = bexk::mlp_factor(&snet, wf, lane);
+  if (lane < 32 && r < n) out[r] = y;
 }
 
 // Logical thread index of (sid, uid) in the reference layout (Interface.cuh:875-879).
@@ -276,7 +307,7 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
       (d.CCI != 0 && d.CCI != 1) || (d.Algorithm != BEX_RK4 && d.Algorithm != BEX_RKCK45))
     return fail(nullptr, BEX_E_ARGUMENT, "bex_create: invalid problem description");
   KernelChoice kc;
-  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, &kc))
+  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
                 "bex_create: units per system must be 8, 16, 32 or 64 in this build");
 
@@ -413,6 +444,9 @@ int bex_destroy(bex_solver* s) {
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
   if (s->d_timeline) hipFree(s->d_timeline);
+  if (s->d_net) hipFree(s->d_net);
+  if (s->d_samples) hipFree(s->d_samples);
+  if (s->d_sample_count) hipFree(s->d_sample_count);
   if (s->event) hipEventDestroy(s->event);
   if (s->timer0) hipEventDestroy(s->timer0);
   if (s->timer1) hipEventDestroy(s->timer1);
@@ -440,9 +474,17 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_PriorityStepThreshold: s->PriorityStepThreshold = (int)v; break;
     case BEX_KernelLayout: {
       KernelChoice kc;
-      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, &kc))
+      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, &kc))
         return fail(s, BEX_E_UNSUPPORTED, "SolverOption: no kernel layout for this unit count");
       s->KernelLayout = (int)v;
+      break;
+    }
+    case BEX_WarmStart: {
+      KernelChoice kc;
+      if (v != 0.0 && !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kLearned, &kc))
+        return fail(s, BEX_E_UNSUPPORTED,
+                    "SolverOption: the learned warm start needs 32 units and the role-pair layout");
+      s->WarmStart = (v != 0.0) ? 1 : 0;
       break;
     }
     default:
@@ -740,8 +782,11 @@ int bex_solve(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
   KernelChoice kc;
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, &kc))
-    return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count and layout");
+  const int mode = s->WarmStart ? bexk::kLearned : (s->d_samples ? bexk::kSampling : bexk::kPlain);
+  if (mode == bexk::kLearned && !s->d_net)
+    return fail(s, BEX_E_ARGUMENT, "Solve: warm start enabled but no network was set");
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, &kc))
+    return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count, layout and mode");
   const int ND = s->d.UD;
   BexKernelArgs a{};
   a.NS = s->d.NS;
@@ -782,6 +827,11 @@ int bex_solve(bex_solver* s) {
   a.BiCGResidualTolerance = s->BiCGResidualTolerance;
   a.BiCGMaxIterations = s->BiCGMaxIterations;
   a.PriorityStepThreshold = s->PriorityStepThreshold > 0 ? s->PriorityStepThreshold : 0x3fffffff;
+  a.WarmStart = s->d_net;
+  a.Samples = s->d_samples;
+  a.SampleCount = s->d_sample_count;
+  a.SampleCapacity = s->SampleCapacity;
+  a.SampleEvery = s->SampleEvery > 0 ? s->SampleEvery : 1;
   a.ref_spb = s->d.SPB;
   a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
@@ -871,6 +921,83 @@ int bex_reset_stats(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
   BEX_HIP(s, hipMemsetAsync(s->d_stats, 0, sizeof(long long) * (size_t)s->d.NS * BEX_NSTAT, s->stream));
+  return BEX_OK;
+}
+
+int bex_mlp_predict_mfma(const float* rows, int stride, long long n, const float* params,
+                         float* out, void* stream) {
+  if (!rows || !params || !out || n <= 0 || stride < BEX_MLP_IN) return BEX_E_ARGUMENT;
+  const long long nb = (n + 31) / 32;
+  if (nb > 0x7fffffff) return BEX_E_UNSUPPORTED;
+  BexMlpDevice host;
+  bex_mlp_pack_device(params, &host);
+  BexMlpDevice* d = nullptr;
+  if (hipMalloc((void**)&d, sizeof(BexMlpDevice)) != hipSuccess) return BEX_E_MEMORY;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int rc = BEX_OK;
+  if (hipMemcpy(d, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice) != hipSuccess) rc = BEX_E_HIP;
+  if (rc == BEX_OK) {
+    hipLaunchKernelGGL(mlp_mfma_kernel, dim3((unsigned)nb), dim3(64), 0, st, rows, stride, n, d, out);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = BEX_E_HIP;
+  }
+  hipFree(d);
+  return rc;
+}
+
+int bex_set_warm_start_network(bex_solver* s, const float* params) {
+  if (!s || !params) return BEX_E_ARGUMENT;
+  for (int k = 0; k < BEX_MLP_NPARAM; k++)
+    if (!std::isfinite(params[k]))
+      return fail(s, BEX_E_ARGUMENT, "bex_set_warm_start_network: non-finite parameter");
+  BEX_HIP(s, hipSetDevice(s->device));
+  BexMlpDevice host;
+  bex_mlp_pack_device(params, &host);
+  if (!s->d_net) BEX_HIP(s, hipMalloc((void**)&s->d_net, sizeof(BexMlpDevice)));
+  // Synchronous: the host staging copy goes out of scope on return.
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  BEX_HIP(s, hipMemcpy(s->d_net, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice));
+  return BEX_OK;
+}
+
+int bex_enable_sampling(bex_solver* s, long long capacity, int every) {
+  if (!s || capacity < 0 || (capacity > 0 && every < 1)) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  if (s->d_samples) BEX_HIP(s, hipFree(s->d_samples));
+  if (s->d_sample_count) BEX_HIP(s, hipFree(s->d_sample_count));
+  s->d_samples = nullptr;
+  s->d_sample_count = nullptr;
+  s->SampleCapacity = 0;
+  s->SampleEvery = 0;
+  if (capacity == 0) return BEX_OK;
+  KernelChoice kc;
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kSampling, &kc))
+    return fail(s, BEX_E_UNSUPPORTED,
+                "bex_enable_sampling: sample collection needs 32 units and the role-pair layout");
+  BEX_HIP(s, hipMalloc((void**)&s->d_samples, sizeof(float) * BEX_SAMPLE_FLOATS * (size_t)capacity));
+  BEX_HIP(s, hipMalloc((void**)&s->d_sample_count, sizeof(unsigned long long)));
+  BEX_HIP(s, hipMemsetAsync(s->d_sample_count, 0, sizeof(unsigned long long), s->stream));
+  s->SampleCapacity = capacity;
+  s->SampleEvery = every;
+  return BEX_OK;
+}
+
+int bex_get_samples(bex_solver* s, float* out, long long max_rows, long long* rows) {
+  if (!s || !rows || (max_rows > 0 && !out)) return BEX_E_ARGUMENT;
+  if (!s->d_samples) return fail(s, BEX_E_ARGUMENT, "bex_get_samples: sampling not enabled");
+  BEX_HIP(s, hipSetDevice(s->device));
+  unsigned long long reserved = 0;
+  BEX_HIP(s, hipMemcpyAsync(&reserved, s->d_sample_count, sizeof reserved, hipMemcpyDeviceToHost,
+                            s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  long long n = (long long)reserved;
+  if (n > s->SampleCapacity) n = s->SampleCapacity - s->SampleCapacity % 32;
+  *rows = n;
+  if (max_rows > 0 && n > 0) {
+    const long long m = n < max_rows ? n : max_rows;
+    BEX_HIP(s, hipMemcpy(out, s->d_samples, sizeof(float) * BEX_SAMPLE_FLOATS * (size_t)m,
+                         hipMemcpyDeviceToHost));
+  }
   return BEX_OK;
 }
 

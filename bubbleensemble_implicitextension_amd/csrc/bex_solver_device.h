@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bex_detmath.h"
+#include "bex_mlp.h"
 #include "bex_types.h"
 
 #ifndef BEX_SYSTEM_DEFINITION
@@ -47,6 +48,13 @@ namespace bexk {
 constexpr int kWave = 64;
 
 enum Algorithm { kRK4 = 0, kRKCK45 = 1 };
+
+// BiCG initial guess and sample collection of a kernel instantiation.
+enum WarmStartMode {
+  kPlain = 0,    // x0 = uncoupled acceleration (the reference's choice)
+  kSampling = 1, // plain, and every SampleEvery-th stage records (features, x / b, x, b)
+  kLearned = 2   // x0 = b * network(features); plain restart if that solve does not converge
+};
 
 // System-scope values the user hooks receive by reference (the reference keeps them in
 // shared memory, Solver.cuh:51-74).
@@ -95,6 +103,7 @@ struct GroupLayout {
   static constexpr int kBlock = kWave * kWaves;
   static constexpr int kSystemsPerBlock = kSystemsPerWave * kWaves;
   static constexpr int kWavesPerEU = 4;  // <= 128 VGPRs
+  static constexpr int kMode = kPlain;
 
   static constexpr size_t lds_bytes() {
     return sizeof(double) * 2 * N * N + sizeof(double2) * kSystemsPerBlock * N +
@@ -183,8 +192,9 @@ struct GroupLayout {
 
   // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
   // (Steppers.cuh:489-561). Returns x in F[1].
+  template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
-                                              const BexKernelArgs& a) const {
+                                              const BexKernelArgs& a, Feat&&, bool) const {
     const double x1sq = x1 * x1;
     const double xu = F[1];
     wave_sync();
@@ -334,21 +344,85 @@ struct FmaxOp {
   }
 };
 
+typedef short bex_v4s __attribute__((ext_vector_type(4)));
+typedef float bex_v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bex_v4s pack_bf16x4(float a, float b, float c, float d) {
+  bex_v4s r;
+  r[0] = (short)bex_bf16_bits(a);
+  r[1] = (short)bex_bf16_bits(b);
+  r[2] = (short)bex_bf16_bits(c);
+  r[3] = (short)bex_bf16_bits(d);
+  return r;
+}
+__device__ __forceinline__ float relu(float z) { return z > 0.0f ? z : 0.0f; }
+
+// Guess factor y of the warm-start network (definition in bex_mlp.h) for the lane's unit, on
+// both halves of a role-pair wave. The lane's half supplies inputs 4h..4h+3 of the first MFMA
+// (features 0-3 on the primal half; feature 4, the bias input 1 and two zeros on the shadow
+// half). Hidden layers run on v_mfma_f32_32x32x8_bf16 with the layer-1 accumulators feeding
+// layer 2's B operand in place; the output layer is a VALU dot over the lane's 32 hidden
+// values plus the partner half's 32.
+__device__ __forceinline__ float mlp_factor(const BexMlpDevice* net, const double* wf, int lane) {
+  const int h = lane >> 5;
+  const bex_v4s bin = (h == 0) ? pack_bf16x4((float)wf[0], (float)wf[1], (float)wf[2], (float)wf[3])
+                               : pack_bf16x4((float)wf[4], 1.0f, 0.0f, 0.0f);
+  bex_v16f zero;
+#pragma unroll
+  for (int r = 0; r < 16; r++) zero[r] = 0.0f;
+  bex_v16f d1[2], d2[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const bex_v4s a1 = __builtin_bit_cast(bex_v4s, net->a1[t][lane]);
+    d1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, bin, zero, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; r++) d2[t][r] = net->b2[32 * t + 8 * (r >> 2) + 4 * h + (r & 3)];
+  }
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const bex_v16f& src = d1[s >> 2];
+    const int q = 4 * (s & 3);
+    const bex_v4s bs = pack_bf16x4(relu(src[q]), relu(src[q + 1]), relu(src[q + 2]), relu(src[q + 3]));
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const bex_v4s a2 = __builtin_bit_cast(bex_v4s, net->a2[t][s][lane]);
+      d2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, bs, d2[t], 0, 0, 0);
+    }
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      part = fmaf(net->w3[32 * t + 8 * (r >> 2) + 4 * h + (r & 3)],
+                  bex_bf16_round(relu(d2[t][r])), part);
+  const float other = __shfl_xor(part, 32, kWave);
+  const float p0 = (h == 0) ? part : other;
+  const float p1 = (h == 0) ? other : part;
+  return net->b3 + (p0 + p1);
+}
+
+template <int MODE>
 struct RolePairLayout32 {
   static constexpr int kUnits = 32;
   static constexpr int kSystemsPerWave = 1;
   static constexpr int kWaves = 1;
   static constexpr int kBlock = kWave;
   static constexpr int kSystemsPerBlock = 1;
-  static constexpr int kWavesPerEU = BEX_ROLEPAIR_WAVES_PER_EU;
+  // The learned kernel needs ~70 more registers for the MFMA tiles: 2 waves per SIMD.
+  static constexpr int kWavesPerEU = (MODE == kLearned) ? 2 : BEX_ROLEPAIR_WAVES_PER_EU;
+  static constexpr int kMode = MODE;
 
-  static constexpr size_t lds_bytes() { return sizeof(Sys) + sizeof(int) * BEX_NSTAT; }
+  static constexpr size_t lds_bytes() {
+    return (MODE == kLearned ? sizeof(BexMlpDevice) : 0) + sizeof(Sys) + sizeof(int) * BEX_NSTAT;
+  }
 
   int lane;
   bool primal_;
   double mrow[32];  // row i of M (primal) or of M^T (shadow)
   Sys* sys;
   int* stat;
+  const BexMlpDevice* net;
 
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     lane = threadIdx.x;
@@ -357,7 +431,16 @@ struct RolePairLayout32 {
     const double* src = primal_ ? a.CouplingMatrix : a.CouplingMatrixT;
 #pragma unroll
     for (int j = 0; j < 32; j++) mrow[j] = src[i + j * 32];
-    sys = reinterpret_cast<Sys*>(smem);
+    unsigned char* p = smem;
+    if (MODE == kLearned) {
+      BexMlpDevice* n = reinterpret_cast<BexMlpDevice*>(p);
+      const uint4* from = reinterpret_cast<const uint4*>(a.WarmStart);
+      uint4* to = reinterpret_cast<uint4*>(n);
+      for (int k = lane; k < (int)(sizeof(BexMlpDevice) / sizeof(uint4)); k += kWave) to[k] = from[k];
+      net = n;
+      p += sizeof(BexMlpDevice);
+    }
+    sys = reinterpret_cast<Sys*>(p);
     stat = reinterpret_cast<int*>(sys + 1);
     if (lane < BEX_NSTAT) stat[lane] = 0;
     wave_sync();
@@ -397,21 +480,17 @@ struct RolePairLayout32 {
     return tree32(lo * up, AddOp());
   }
 
-  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
-  // (Steppers.cuh:489-561). Returns x in F[1] on both halves.
-  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
-                                              const BexKernelArgs& a) const {
-    const double x1sq = x1 * x1;
-    const double xu = F[1];
-    // b = rD (M CPT) + xu and r0 = b - A xu, meaningful on the primal half.
-    const double b = fma(rD, matvec(cpt), xu);
-    const double ax = fma(rD, matvec(x1sq * xu), xu);
+  // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
+  // meaningful on the primal half.
+  __device__ __forceinline__ BicgOut bicg(double b, double x0, double x1sq, double rD,
+                                          const BexKernelArgs& a, double* xout) const {
+    const double ax = fma(rD, matvec(x1sq * x0), x0);
     double lo, up;
     halves(-ax + b, &lo, &up);
     double res = lo;  // r on the primal half, r~ = r on the shadow half
     const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
     const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
-    double dir = 0.0, x = xu, rho2 = 0.0;
+    double dir = 0.0, x = x0, rho2 = 0.0;
     double rho1 = dot_halves(res);  // r . r~
     BicgOut out{0, false, true};
     for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
@@ -443,8 +522,65 @@ struct RolePairLayout32 {
         break;
       }
     }
+    *xout = x;
+    return out;
+  }
+
+  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT) (Steppers.cuh:489-561).
+  // Returns x in F[1] on both halves. feat(crs, wf) evaluates the warm-start features of the
+  // lane's unit given the coupling row sum crs (learned and sampling kernels only).
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&& feat,
+                                              bool sample_now) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    const double b = fma(rD, matvec(cpt), xu);  // meaningful on the primal half
+    double lo, up;
+    double wf[BEX_MLP_IN];
+    double bl = 0.0;
+    const bool need_features = (MODE == kLearned) || (MODE == kSampling && sample_now);
+    if (need_features) {
+      halves(matvec(x1sq), &lo, &up);  // sum_j M_ij x1_j^2 of the lane's unit, both halves
+      feat(lo, wf);
+      halves(b, &bl, &up);             // b_i on both halves
+    }
+    double x0 = xu;
+    if (MODE == kLearned) {
+      const float y = mlp_factor(net, wf, lane);
+      if (bex::is_finite((double)y) && y <= BEX_MLP_TARGET_CLAMP && y >= -BEX_MLP_TARGET_CLAMP)
+        x0 = bl * (double)y;
+    }
+    double x;
+    BicgOut out = bicg(b, x0, x1sq, rD, a, &x);
+    if (MODE == kLearned && out.maxit) {  // plain restart from the uncoupled acceleration
+      const BicgOut again = bicg(b, xu, x1sq, rD, a, &x);
+      if (lane == 0) stat[BEX_STAT_WARM_FALLBACKS]++;
+      out.iterations += again.iterations;
+      out.rho_exit = again.rho_exit;
+      out.maxit = again.maxit;
+    }
     halves(x, &lo, &up);
     F[1] = lo;
+    if (MODE == kSampling && sample_now) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(a.SampleCount, 32ull);
+      base = __builtin_amdgcn_readfirstlane((unsigned)base) |
+             ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32);
+      if (primal_ && base + 32 <= (unsigned long long)a.SampleCapacity) {
+        float* row = a.Samples + (base + (unsigned long long)lane) * BEX_SAMPLE_FLOATS;
+        double t = lo / bl;
+        t = (t > BEX_MLP_TARGET_CLAMP) ? BEX_MLP_TARGET_CLAMP : t;
+        t = (t < -BEX_MLP_TARGET_CLAMP) ? -BEX_MLP_TARGET_CLAMP : t;
+#pragma unroll
+        for (int k = 0; k < BEX_MLP_IN; k++) row[k] = (float)wf[k];
+        row[5] = bex::is_finite(t) ? (float)t : 1.0f;
+        row[6] = (float)lo;
+        row[7] = (float)bl;
+        row[8] = (float)xu;
+        row[9] = 0.0f;
+      }
+    }
     return out;
   }
 };
@@ -562,6 +698,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
 
   int terminate = 0;
   int accepted = 0;
+  long long stage_count = 0;  // RK stages of this launch (sample selection)
   while (terminate == 0) {  // Solver.cuh:329-572
     int updateStep = 1, endReached = 0;
     dt = dtNew;
@@ -629,7 +766,17 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
                                                   a.GlobalParameters,
                                                   a.IntegerGlobalParameters, uacc, iuacc,
                                                   S.sACC, S.isACC, &cpt, &cpf);
-      const BicgOut o = lay.coupling(F, xs[0], cpt, cpf, a);
+      bool sample_now = false;
+      if (L::kMode == kSampling) {
+        sample_now = ((stage_count + sid) % a.SampleEvery) == 0;
+        stage_count++;
+      }
+      auto features = [&](double crs, double* wf) {
+        CoupledSystems_PerBlock_WarmStartFeatures<double>(sid, uid, wf, Ts, dt, xs, upar, S.sPAR,
+                                                          a.GlobalParameters,
+                                                          a.IntegerGlobalParameters, crs);
+      };
+      const BicgOut o = lay.coupling(F, xs[0], cpt, cpf, a, features, sample_now);
       if (leader) {
         st[BEX_STAT_STAGES]++;
         st[BEX_STAT_BICG_ITERATIONS] += o.iterations;

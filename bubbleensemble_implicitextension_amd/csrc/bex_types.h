@@ -3,6 +3,8 @@
 
 #include <stdint.h>
 
+struct BexMlpDevice;
+
 // Per-system counters (same layout as the CPU oracle's).
 enum BexStat {
   BEX_STAT_ACCEPTED = 0,
@@ -17,6 +19,7 @@ enum BexStat {
   BEX_STAT_EVENTS_OMITTED = 9,
   BEX_STAT_ATTEMPTS = 10,
   BEX_STAT_HIST0 = 11,  // 20 bins: stages that ran k = 0..19 BiCG iterations (19 = 19 or more)
+  BEX_STAT_WARM_FALLBACKS = 31,  // learned initial guess rejected or not converged: restarted
   BEX_NSTAT = 32
 };
 
@@ -65,4 +68,11 @@ struct BexKernelArgs {
   double BiCGResidualTolerance;
   int BiCGMaxIterations;
   int PriorityStepThreshold;  // accepted steps after which a system's wave priority rises
+
+  // BiCG warm start and training-sample collection (no reference counterpart).
+  const BexMlpDevice* WarmStart;     // network in the solver's operand layout (learned kernel)
+  float* Samples;                    // [SampleCapacity][BEX_SAMPLE_FLOATS]
+  unsigned long long* SampleCount;   // rows reserved so far (may exceed the capacity)
+  long long SampleCapacity;
+  int SampleEvery;                   // a system records every SampleEvery-th RK stage
 };

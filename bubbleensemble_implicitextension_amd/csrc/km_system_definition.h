@@ -114,6 +114,30 @@ __forceinline__ __device__ __host__ void CoupledSystems_PerBlock_Initialization(
   if (uid == 0) T = TD[0];
 }
 
+// Warm-start features of one unit for the learned BiCG initial guess (no reference
+// counterpart; only called when the warm start or training-sample collection is enabled).
+// CRS is the row sum of the implicit operator's coupling, sum_j M_ij X_j[0]^2, supplied by the
+// solver. Each feature is scaled to O(1) over the solver's operating range:
+//   WF[0] = log(R/R_E), WF[1] = 10 Rdot/c_L, WF[2] = forcing pressure / C0 (both frequencies),
+//   WF[3] = log(1 + CRS), WF[4] = log(dT) / 10.
+template <class Precision>
+__forceinline__ __device__ __host__ void CoupledSystems_PerBlock_WarmStartFeatures(
+    int sid, int uid, Precision* WF, Precision T, Precision dT, Precision* X, Precision* uPAR,
+    Precision* sPAR, Precision* gPAR, int* igPAR, Precision CRS) {
+  Precision lo;
+  Precision s1;
+  Precision c1;
+  bex::sincospi(2.0 * T, &s1, &c1);
+  Precision s2;
+  Precision c2;
+  bex::sincos(2.0 * uPAR[11] * PI * T + uPAR[12], &s2, &c2);
+  WF[0] = bex::log_dd(X[0], &lo);
+  WF[1] = 10.0 * uPAR[9] * X[1];
+  WF[2] = (uPAR[5] * s1 + uPAR[6] * s2) / uPAR[0];
+  WF[3] = bex::log_dd(1.0 + CRS, &lo);
+  WF[4] = 0.1 * bex::log_dd(dT, &lo);
+}
+
 // BubbleEnsemble_SystemDefinition.cuh:118-126
 template <class Precision>
 __forceinline__ __device__ __host__ void CoupledSystems_PerBlock_Finalization(

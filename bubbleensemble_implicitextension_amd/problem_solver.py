@@ -63,9 +63,12 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     BiCGMaxIterations = 16
     KernelLayout = 17
     PriorityStepThreshold = 18
+    WarmStart = 19
 
 
 LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR = 0, 1, 2
+MLP_NPARAM = 4609  # warm-start network parameters (csrc/bex_mlp.h)
+SAMPLE_FLOATS = 10  # floats per training-sample row
 
 
 # Reference-style bare names (the reference uses unscoped enums).
@@ -238,6 +241,30 @@ class ProblemSolver:
 
     def reset_stats(self):
         self._check(self._lib.bex_reset_stats(self._h))
+
+    # -- BiCG warm start (no reference counterpart) ---------------------------------------
+    def set_warm_start_network(self, params):
+        """Uploads a trained network (4609 FP32 parameters, layout of csrc/bex_mlp.h)."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32).reshape(-1))
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"expected {MLP_NPARAM} parameters, got {p.size}")
+        self._check(self._lib.bex_set_warm_start_network(
+            self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def enable_sampling(self, capacity, every):
+        self._check(self._lib.bex_enable_sampling(self._h, int(capacity), int(every)))
+
+    def samples(self):
+        """Collected training rows [n, 10]: 5 features, x/b (clamped), x, b, uncoupled
+        acceleration, 0."""
+        n = ctypes.c_longlong()
+        self._check(self._lib.bex_get_samples(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, SAMPLE_FLOATS), dtype=np.float32)
+        if n.value:
+            self._check(self._lib.bex_get_samples(
+                self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n.value,
+                ctypes.byref(n)))
+        return out
 
     # -- convenience views (reference layout with TPB == SPB*UPS) --------------------------
     def state(self):

@@ -83,7 +83,11 @@ enum bex_option {
   BEX_KernelLayout,
   /* accepted steps (per launch) after which a system's wave priority rises; 0 disables
    * (no reference counterpart; default 3000) */
-  BEX_PriorityStepThreshold
+  BEX_PriorityStepThreshold,
+  /* 1: BiCG starts from the learned guess x0 = b * y(features) instead of the uncoupled
+   * acceleration (Steppers.cuh:494-499), with a restart from the uncoupled acceleration when
+   * that solve does not converge; needs bex_set_warm_start_network. 32 units only. */
+  BEX_WarmStart
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32 units, lane groups otherwise */
@@ -163,6 +167,37 @@ int bex_timer_stop(bex_solver* s, double* elapsed_ms);
  * it and returns the counter frequency. No reference counterpart. */
 int bex_enable_timeline(bex_solver* s, int on);
 int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second);
+
+/* BiCG warm start (no reference counterpart). params: BEX_MLP_NPARAM (4609) floats in the
+ * layout of bubbleensemble_implicitextension_amd/csrc/bex_mlp.h (W1[6][64], W2[64][64], b2[64],
+ * w3[64], b3); copied, converted to the kernel's bf16 operand layout and uploaded. */
+int bex_set_warm_start_network(bex_solver* s, const float* params);
+/* Training-sample collection: while enabled (capacity > 0) and the warm start is off, every
+ * `every`-th RK stage of each system appends 32 rows of 10 floats (5 warm-start features, the
+ * target x/b clamped to [-5, 5], x, b, the uncoupled acceleration, 0) until `capacity` rows
+ * are filled. capacity 0 disables. */
+int bex_enable_sampling(bex_solver* s, long long capacity, int every);
+/* Copies up to max_rows collected rows to out (host) and returns the number collected. */
+int bex_get_samples(bex_solver* s, float* out, long long max_rows, long long* rows);
+
+/* Warm-start network training on the device (no reference counterpart). All pointers are
+ * device pointers; stream is a hipStream_t (NULL = default stream); calls are asynchronous.
+ * rows: n samples of `stride` floats, features in columns 0-4, target in column 5.
+ * bex_mlp_loss_grad writes the mean squared error to loss[0] and its gradient with respect
+ * to the 4609 parameters to grad, using workspace (bex_mlp_workspace_floats(n) floats). */
+long long bex_mlp_workspace_floats(long long n);
+int bex_mlp_loss_grad(const float* rows, int stride, long long n, const float* params,
+                      float* grad, float* loss, float* workspace, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on the 4609 parameters. */
+int bex_mlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr,
+                 float beta1, float beta2, float eps, void* stream);
+/* Network output y for n rows (FP32 VALU, the training forward pass). */
+int bex_mlp_predict(const float* rows, int stride, long long n, const float* params, float* out,
+                    void* stream);
+/* Network output y for n rows through the solver's BF16 MFMA inference path; params is a host
+ * pointer (4609 floats), rows/out device pointers; synchronous. */
+int bex_mlp_predict_mfma(const float* rows, int stride, long long n, const float* params,
+                         float* out, void* stream);
 
 /* Message of the last failing call on this handle (or of the last bex_create failure when
  * s is NULL). */

@@ -1,0 +1,169 @@
+"""GPU tests of the learned BiCG warm start: training kernels against the host oracle, the
+solver's MFMA inference against the oracle forward pass, sample collection, and the ablation
+(trained network vs the plain initial guess vs an untrained network) on held-out systems."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(n, seed):
+    rng = np.random.default_rng(seed)
+    rows = np.zeros((n, 10), dtype=np.float32)
+    rows[:, :5] = rng.normal(0.0, 1.0, (n, 5))
+    rows[:, 5] = rng.normal(1.0, 0.3, n)
+    return rows
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = mo.init_params(seed).astype(np.float64) + rng.normal(0.0, 0.05, mo.NPARAM)
+    return p.astype(np.float32)
+
+
+def test_gradient_kernel_matches_oracle():
+    import torch
+    from bubbleensemble_implicitextension_amd import _native
+    from bubbleensemble_implicitextension_amd.warm_start import _ptr, _stream
+
+    print("case: bex_mlp_loss_grad, n=1000 (ragged last block)", flush=True)
+    lib = _native.load()
+    rows = _rows(1000, 11)
+    p = _params(12)
+    r = torch.from_numpy(rows).cuda()
+    pd = torch.from_numpy(p).cuda()
+    g = torch.zeros_like(pd)
+    loss = torch.zeros(1, device="cuda")
+    ws = torch.zeros(int(lib.bex_mlp_workspace_floats(1000)), device="cuda")
+    _native.check(lib.bex_mlp_loss_grad(_ptr(r), 10, 1000, _ptr(pd), _ptr(g), _ptr(loss),
+                                        _ptr(ws), _stream(torch)))
+    torch.cuda.synchronize()
+    lo, go = mo.loss_grad(p, rows)
+    assert abs(loss.item() - lo) <= 1e-4 * lo
+    gg = g.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(gg - go)) <= 2e-3 * np.max(np.abs(go)), np.max(np.abs(gg - go))
+    # Every parameter block carries gradient.
+    for a, b in [(0, 384), (384, 4480), (4480, 4544), (4544, 4608)]:
+        assert np.max(np.abs(gg[a:b])) > 0
+
+
+def test_adam_step_matches_formula():
+    import torch
+    from bubbleensemble_implicitextension_amd import _native
+    from bubbleensemble_implicitextension_amd.warm_start import _ptr, _stream
+
+    print("case: bex_mlp_adam, two steps", flush=True)
+    lib = _native.load()
+    rng = np.random.default_rng(2)
+    p0 = rng.normal(0, 1, mo.NPARAM).astype(np.float32)
+    grads = [rng.normal(0, 1, mo.NPARAM).astype(np.float32) for _ in range(2)]
+    p = torch.from_numpy(p0.copy()).cuda()
+    m = torch.zeros_like(p)
+    v = torch.zeros_like(p)
+    pr, mr, vr = p0.astype(np.float64), np.zeros(mo.NPARAM), np.zeros(mo.NPARAM)
+    for t, gr in enumerate(grads, start=1):
+        g = torch.from_numpy(gr).cuda()
+        _native.check(lib.bex_mlp_adam(_ptr(p), _ptr(g), _ptr(m), _ptr(v), t, 1e-2, 0.9, 0.999,
+                                       1e-8, _stream(torch)))
+        mr = 0.9 * mr + 0.1 * gr
+        vr = 0.999 * vr + 0.001 * gr.astype(np.float64) ** 2
+        pr = pr - 1e-2 * (mr / (1 - 0.9 ** t)) / (np.sqrt(vr / (1 - 0.999 ** t)) + 1e-8)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(p.cpu().numpy(), pr, rtol=1e-5, atol=1e-6)
+
+
+def test_inference_paths_match_oracle():
+    from bubbleensemble_implicitextension_amd.warm_start import predict
+
+    print("case: bex_mlp_predict and bex_mlp_predict_mfma, n=1000", flush=True)
+    rows = _rows(1000, 21)
+    p = _params(22)
+    yo, _ = mo.forward(p, rows)
+    y32 = predict(p, rows, mfma=False).astype(np.float64)
+    ymf = predict(p, rows, mfma=True).astype(np.float64)
+    scale = 1.0 + np.abs(yo)
+    # FP32 accumulation (VALU or MFMA order) against float64: rare 1-ulp bf16 flips of a hidden
+    # activation move y by ~1e-3 at most.
+    assert np.max(np.abs(y32 - yo) / scale) < 3e-3
+    assert np.max(np.abs(ymf - yo) / scale) < 3e-3
+    assert np.median(np.abs(ymf - yo) / scale) < 1e-5
+    assert np.median(np.abs(y32 - yo) / scale) < 1e-5
+
+
+def _run(ens, network=None, res_tol=1e-13, maxit=60):
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.SolverOption(O.BiCGResidualTolerance, res_tol)
+    s.SolverOption(O.BiCGMaxIterations, maxit)
+    if network is not None:
+        s.set_warm_start_network(network)
+        s.SolverOption(O.WarmStart, 1)
+    s.SynchroniseFromHostToDevice(V.All)
+    s.Solve()
+    s.SynchroniseFromDeviceToHost(V.All)
+    s.SynchroniseSolver()
+    out = (s.state().copy(), s.accessories().copy(), s.event_counts().copy(), s.stats().copy())
+    s.close()
+    return out
+
+
+def test_learned_warm_start_ablation():
+    """Samples from a held-out ensemble train the network on the GPU; on other systems the
+    trained network's initial guess beats both the plain guess and an untrained network, and
+    the learned solve reproduces the plain solve."""
+    from bubbleensemble_implicitextension_amd import warm_start as wsm
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    print("case: sample collection, 64 held-out systems", flush=True)
+    train_ens = wl.config1(seed=wsm.TRAIN_SEED)
+    sub = train_ens.subset(np.arange(0, train_ens.NS, train_ens.NS // 64))
+    rows = wsm.collect_samples(sub, capacity=300_000, every=16)
+    assert rows.shape[0] > 50_000
+    assert np.all(np.isfinite(rows[:, :6]))
+    # The target column is x / b of the recorded solve, clamped.
+    ok = np.abs(rows[:, 7]) > 1e-30
+    t = np.clip(rows[ok, 6].astype(np.float64) / rows[ok, 7], -5, 5)
+    np.testing.assert_allclose(rows[ok, 5], t, rtol=1e-5, atol=1e-6)
+
+    print("case: training, 3 epochs", flush=True)
+    trainer = wsm.DeviceTrainer(rows, wsm.init_params(0), batch=4096, lr=2e-3)
+    first = trainer.full_loss()
+    for _ in range(3):
+        trainer.epoch()
+    last = trainer.full_loss()
+    var = float(np.var(rows[:, 5].astype(np.float64)))
+    print(f"training-set loss {first:.4f} -> {last:.4f} (target variance {var:.4f})")
+    # The five features explain little of x/b's variance (heavy tails where b crosses zero):
+    # training must lower the loss to at most that of the best constant.
+    assert last < first and last <= 1.01 * var, (first, last, var)
+    trained = trainer.params()
+    untrained = wsm.init_params(0)
+
+    print("case: guess quality on held-out systems", flush=True)
+    evaluation = wl.config1().subset(np.arange(5, 16384, 16384 // 24))
+    held = wsm.collect_samples(evaluation, capacity=100_000, every=16)
+    le, pe = wsm.guess_errors(held, wsm.predict(trained, held, mfma=True))
+    ue, _ = wsm.guess_errors(held, wsm.predict(untrained, held, mfma=True))
+    print(f"median relative guess error: trained {le:.4f}, plain {pe:.4f}, untrained {ue:.4f}")
+    assert le < pe and le < ue
+
+    print("case: learned vs plain solve, residual 1e-13", flush=True)
+    small = evaluation.subset(np.arange(0, evaluation.NS, 3))
+    xp, ap, ep, sp = _run(small)
+    xl, al, el, sl = _run(small, network=trained)
+    it_plain = sp[:, 3].sum() / sp[:, 2].sum()
+    it_learned = sl[:, 3].sum() / sl[:, 2].sum()
+    print(f"BiCG iterations per RHS: plain {it_plain:.3f}, learned {it_learned:.3f}, "
+          f"fallbacks {int(sl[:, 31].sum())}")
+    np.testing.assert_array_equal(ep, el)
+    rel = np.abs(xl - xp) / np.maximum(np.abs(xp), 1e-3)
+    assert np.max(rel) <= 1e-6, np.max(rel)
+    assert it_learned <= 1.25 * it_plain
