@@ -89,12 +89,66 @@ def copy_calibration():
     return 0
 
 
+def run_leg(ens, steps, warmup, end_time, network=None):
+    """Times `steps` Solve() calls of the whole ensemble (inputs resident on the device)."""
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    solver = collapse.make_solver(ens)
+    collapse.fill(solver, ens)
+    solver.host_view(V.TimeDomain)[ens.NS:] = end_time
+    if network is not None:
+        solver.set_warm_start_network(network)
+        solver.SolverOption(O.WarmStart, 1)
+    solver.SynchroniseFromHostToDevice(V.All)
+    for _ in range(warmup):
+        solver.Solve()
+    solver.InsertSynchronisationPoint()
+    solver.SynchroniseSolver()
+    solver.reset_stats()
+    solver.SynchroniseDevice()
+
+    t0 = time.perf_counter()
+    solver.timer_start()
+    for _ in range(steps):
+        solver.Solve()
+    kernel_ms = solver.timer_stop()
+    solver.SynchroniseDevice()
+    wall = time.perf_counter() - t0
+
+    solver.SynchroniseFromDeviceToHost(V.All)
+    solver.SynchroniseSolver()
+    out = {"wall": wall, "kernel_ms": kernel_ms, "stats": solver.stats(),
+           "state": solver.state().copy(), "acc": solver.accessories().copy(),
+           "events": solver.event_counts().copy()}
+    solver.close()
+    return out
+
+
+def error_vs(ref, other):
+    """Relative differences of final states and recorded collapse extrema, and event-count
+    agreement, of one leg against another on the same systems."""
+    ok = np.isfinite(ref["state"]) & np.isfinite(other["state"])
+    rs = np.abs(other["state"] - ref["state"]) / np.maximum(np.abs(ref["state"]), 1e-3)
+    same = ref["events"] == other["events"]  # [NS, N] units with identical event counts
+    ra = np.abs(other["acc"] - ref["acc"]) / np.maximum(np.abs(ref["acc"]), 1e-3)
+    ra = ra[same]  # collapse records are comparable where the same events were recorded
+    return {"state_max_rel": float(np.max(rs[ok])), "state_median_rel": float(np.median(rs[ok])),
+            "state_frac_within_1e-6": float(np.mean(rs[ok] <= 1e-6)),
+            "collapse_max_rel_same_events": float(np.max(ra)) if ra.size else None,
+            "collapse_frac_within_1e-6": float(np.mean(ra <= 1e-6)) if ra.size else None,
+            "units_event_counts_identical_frac": float(np.mean(same))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm-start leg")
     ap.add_argument("--end-time", type=float, default=1.0,
                     help="fraction of a period per step (profiling only)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -108,40 +162,25 @@ def main():
               file=sys.stderr)
         return 2
 
-    from bubbleensemble_implicitextension_amd import collapse
     from bubbleensemble_implicitextension_amd import workloads as wl
-    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+    from bubbleensemble_implicitextension_amd import warm_start
 
     ens = wl.config1()
     N = ens.N
-    solver = collapse.make_solver(ens)
-    collapse.fill(solver, ens)
-    solver.host_view(V.TimeDomain)[ens.NS:] = args.end_time
-    solver.SynchroniseFromHostToDevice(V.All)
-    for _ in range(args.warmup):
-        solver.Solve()
-    solver.InsertSynchronisationPoint()
-    solver.SynchroniseSolver()
-    print(f"# warmup done ({args.warmup} periods)", file=sys.stderr, flush=True)
-    solver.reset_stats()
-    solver.SynchroniseDevice()
-
-    t0 = time.perf_counter()
-    solver.timer_start()
-    for k in range(args.steps):
-        solver.Solve()
-    kernel_ms = solver.timer_stop()
-    solver.SynchroniseDevice()
-    wall = time.perf_counter() - t0
-
-    stats = solver.stats()
+    plain = run_leg(ens, args.steps, args.warmup, args.end_time)
+    print(f"# plain leg done ({plain['kernel_ms']:.1f} ms per {args.steps} launches)",
+          file=sys.stderr, flush=True)
+    stats = plain["stats"]
+    kernel_ms = plain["kernel_ms"]
+    wall = plain["wall"]
     flops = flops_fm1(stats, N)
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     iters_per_rhs = float(stats[:, 3].sum()) / max(1.0, float(stats[:, 2].sum()))
     value = ens.NS * args.steps / wall
 
     line = {
-        "metric": "system-periods/s (plain FP64 RKCK45 + BiCG, rtol=atol=1e-10)",
+        "metric": "systems x acoustic periods/s at rtol=atol=1e-10, plain FP64 RKCK45 + BiCG "
+                  "(learned warm start vs plain in 'learned'); BiCG iters/RHS",
         "value": value,
         "unit": "system-periods/s",
         "n_gpus": 1,
@@ -162,10 +201,25 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic("period") if args.end_time == 1.0 else None,
-                     "kernel": "bexk::solve_kernel<RolePairLayout32, RKCK45>",
+                     "kernel": "bexk::solve_kernel<RolePairLayout32<kPlain>, RKCK45>",
                      "kernel_ms_per_launch": kernel_ms / args.steps,
                      "flops_per_launch": flops / args.steps},
     }
+    if not args.no_learned and os.path.exists(warm_start.DEFAULT_NETWORK):
+        net = np.load(warm_start.DEFAULT_NETWORK)
+        learned = run_leg(ens, args.steps, args.warmup, args.end_time, network=net)
+        ls = learned["stats"]
+        lv = ens.NS * args.steps / learned["wall"]
+        line["learned"] = {
+            "value": lv, "unit": "system-periods/s",
+            "ms_per_step": learned["wall"] * 1e3 / args.steps,
+            "kernel_ms_per_launch": learned["kernel_ms"] / args.steps,
+            "bicg_iterations_per_rhs": float(ls[:, 3].sum()) / max(1.0, float(ls[:, 2].sum())),
+            "warm_fallbacks": int(ls[:, 31].sum()),
+            "speedup_vs_plain": lv / value,
+            "network": "5-64-64-1 BF16 MFMA, trained on a held-out-seed ensemble",
+            "error_vs_plain": error_vs(plain, learned),
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

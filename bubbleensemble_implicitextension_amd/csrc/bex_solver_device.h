@@ -89,6 +89,7 @@ struct BicgOut {
   int iterations;
   bool rho_exit;
   bool maxit;
+  double rmax;  // max |r| after the last iteration (role-pair layout; +inf before the first)
 };
 
 // =========================================================================================
@@ -209,7 +210,7 @@ struct GroupLayout {
     double rt = r;
     double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
     double rho1 = sum(r * rt);
-    BicgOut out{0, false, true};
+    BicgOut out{0, false, true, 0.0};
     for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
       if (iter == 1) {
         p = r;
@@ -482,17 +483,22 @@ struct RolePairLayout32 {
 
   // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
   // meaningful on the primal half.
-  __device__ __forceinline__ BicgOut bicg(double b, double x0, double x1sq, double rD,
-                                          const BexKernelArgs& a, double* xout) const {
+  // Initial residual r0 = b - A x0 of the primal half, on both halves.
+  __device__ __forceinline__ double residual(double b, double x0, double x1sq, double rD) const {
     const double ax = fma(rD, matvec(x1sq * x0), x0);
     double lo, up;
     halves(-ax + b, &lo, &up);
-    double res = lo;  // r on the primal half, r~ = r on the shadow half
+    return lo;
+  }
+
+  __device__ __forceinline__ BicgOut bicg(double res0, double x0, double x1sq, double rD,
+                                          const BexKernelArgs& a, double* xout) const {
+    double res = res0;  // r on the primal half, r~ = r on the shadow half
     const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
     const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
     double dir = 0.0, x = x0, rho2 = 0.0;
     double rho1 = dot_halves(res);  // r . r~
-    BicgOut out{0, false, true};
+    BicgOut out{0, false, true, bex::infinity()};
     for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
       if (iter == 1) {
         dir = res;
@@ -517,6 +523,7 @@ struct RolePairLayout32 {
       double rmax = bex::fabs_d(res);
       rho1 = lo2 * up2;
       tree32x2(rmax, rho1, FmaxOp(), AddOp());
+      out.rmax = rmax;
       if (rmax < a.BiCGResidualTolerance) {
         out.maxit = false;
         break;
@@ -546,15 +553,32 @@ struct RolePairLayout32 {
       halves(b, &bl, &up);             // b_i on both halves
     }
     double x0 = xu;
+    double r0 = residual(b, xu, x1sq, rD);
+    bool learned_start = false;  // wave-uniform
+    double mu = 0.0;             // max |r0| of the plain start (learned kernel)
     if (MODE == kLearned) {
+      // The learned start is used only when its initial residual is smaller (max norm) than
+      // the plain start's, so BiCG never starts further from the solution than the reference.
       const float y = mlp_factor(net, wf, lane);
+      double xl = xu;
       if (bex::is_finite((double)y) && y <= BEX_MLP_TARGET_CLAMP && y >= -BEX_MLP_TARGET_CLAMP)
-        x0 = bl * (double)y;
+        xl = bl * (double)y;
+      const double rl = residual(b, xl, x1sq, rD);
+      double ml = bex::fabs_d(rl);
+      mu = bex::fabs_d(r0);
+      tree32x2(mu, ml, FmaxOp(), FmaxOp());
+      if (ml < mu) {
+        learned_start = true;
+        x0 = xl;
+        r0 = rl;
+      }
     }
     double x;
-    BicgOut out = bicg(b, x0, x1sq, rD, a, &x);
-    if (MODE == kLearned && out.maxit) {  // plain restart from the uncoupled acceleration
-      const BicgOut again = bicg(b, xu, x1sq, rD, a, &x);
+    BicgOut out = bicg(r0, x0, x1sq, rD, a, &x);
+    // A learned start that ends unconverged (iteration cap) and no closer than the plain start
+    // began, or non-finite, is discarded: plain restart from the uncoupled acceleration.
+    if (MODE == kLearned && learned_start && out.maxit && !(out.rmax < mu)) {
+      const BicgOut again = bicg(residual(b, xu, x1sq, rD), xu, x1sq, rD, a, &x);
       if (lane == 0) stat[BEX_STAT_WARM_FALLBACKS]++;
       out.iterations += again.iterations;
       out.rho_exit = again.rho_exit;

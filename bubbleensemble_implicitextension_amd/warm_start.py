@@ -24,6 +24,14 @@ import time
 
 import numpy as np
 
+# torch supplies device memory and streams for training. Import it before libbex.so is loaded
+# so that the process has one HIP runtime (torch's); loading libbex first binds the system
+# runtime and torch then reports no device.
+try:
+    import torch as _torch_module  # noqa: F401
+except ImportError:  # training and batch prediction then report that they need torch
+    _torch_module = None
+
 from . import _native
 from . import collapse
 from . import workloads as wl
