@@ -86,6 +86,27 @@ _SIGNATURES = {
     "bex_mlp_predict_mfma": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
                               ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p],
                              ctypes.c_int),
+    "bex_pool_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "bex_pool_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_pool_class_solver": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+                              ctypes.c_int),
+    "bex_pool_set_couplings": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_pool_set_order": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                           ctypes.c_int),
+    "bex_pool_order_by_cost": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_pool_solve": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_pool_solve_periods": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_pool_synchronise": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_pool_variant_count": ([ctypes.c_int], ctypes.c_int),
+    "bex_pool_variant_name": ([ctypes.c_int, ctypes.c_int], ctypes.c_char_p),
+    "bex_pool_set_variant": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "bex_pool_get_variant": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                             ctypes.c_int),
+    "bex_pool_autotune": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double)],
+                          ctypes.c_int),
+    "bex_pool_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
     "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
 }
 
@@ -113,6 +134,13 @@ class BexError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(f"[{code}] {message}")
         self.code = code
+
+
+def pool_check(code, handle=None):
+    if code != 0:
+        msg = load().bex_pool_last_error(handle)
+        raise BexError(code, msg.decode() if msg else "")
+    return code
 
 
 def check(code, handle=None):

@@ -34,7 +34,8 @@ def fill(solver, ens):
     solver.host_view(V.CouplingIndex)[:] = 1
     reset_accessories(solver)
     if ens.coupling.ndim != 2:
-        raise ValueError("ProblemSolver takes one coupling matrix for all systems")
+        apply_options(solver, ens.options)
+        return  # per-system matrices: ProblemPool.set_couplings
     N = ens.N
     for row in range(N):
         for col in range(N):
@@ -90,3 +91,17 @@ def write_collapse_file(solver, path, ncoll=wl.NCOLL):
                 fields += [f"{v:10.3e}" for v in acc[sid, :, c]]
                 fields += [f"{v:10.3e}" for v in acc[sid, :, c + ncoll]]
                 f.write(",".join(fields) + ",\n")
+
+
+def make_pool(ensembles, device=0):
+    """ProblemPool with one size class per Ensemble (per-system coupling matrices), filled."""
+    from .problem_solver import ProblemPool
+
+    pool = ProblemPool([e.N for e in ensembles], [e.NS for e in ensembles], NUP=wl.NUP,
+                       NUA=wl.NUA, NiUA=wl.NIUA, NE=1, device=device)
+    for c, e in enumerate(ensembles):
+        s = pool.solver(c)
+        fill(s, e)
+        M = e.coupling if e.coupling.ndim == 3 else np.broadcast_to(e.coupling, (e.NS, e.N, e.N))
+        pool.set_couplings(c, M)
+    return pool

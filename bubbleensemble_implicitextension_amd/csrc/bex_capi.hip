@@ -8,8 +8,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/bex_capi.h"
 #include "bex_solver_device.h"
@@ -46,6 +48,8 @@ struct bex_solver {
   double* h_CouplingOriginal = nullptr;  // full [c][N][N], host only (Interface.cuh:1076)
   double* d_CouplingFull = nullptr;      // full [c][N][N] expanded from the compact storage
   double* d_CouplingFullT = nullptr;     // transpose of coupling 0
+  bool PerSystemCoupling = false;        // pool classes: one full matrix per system
+  double* d_CouplingSys = nullptr;       // [NS][N][N] column-major (pool classes)
   double* h_tol = nullptr;               // [2*UD + NE] rtol, atol, event tolerance
   int* h_dir = nullptr;                  // [NE]
   double* d_tol = nullptr;
@@ -67,6 +71,7 @@ struct bex_solver {
   long long SampleCapacity = 0;
   int SampleEvery = 0;
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
+  int config_epoch = 0;  // bumped by every option change (pools re-capture their graph)
   std::string err;
 };
 
@@ -100,7 +105,7 @@ int bound(bex_solver* s, const char* fn, const char* what, long long v, long lon
   return BEX_OK;
 }
 
-using KernelFn = void (*)(BexKernelArgs);
+using KernelFn = void (*)(BexKernelArgs, long long);
 
 struct KernelChoice {
   KernelFn fn;
@@ -134,6 +139,7 @@ bool kernel_for(int ups, int alg, int layout, int mode, KernelChoice* k) {
     case 16: *k = choice<bexk::GroupLayout<16>>(alg); return true;
     case 32: *k = choice<bexk::GroupLayout<32>>(alg); return true;
     case 64: *k = choice<bexk::GroupLayout<64>>(alg); return true;
+    case 128: *k = choice<bexk::WideLayout<128, false>>(alg); return true;
     default: return false;
   }
 }
@@ -155,6 +161,57 @@ __global__ __launch_bounds__(64) void mlp_mfma_kernel(const float* __restrict__ 
   for (int k = 0; k < BEX_MLP_IN; k++) wf[k] = (r < n) ? (double)rows[r * stride + k] : 0.0;
   const float y = bexk::mlp_factor(&snet, wf, lane);
   if (lane < 32 && r < n) out[r] = y;
+}
+
+// Kernel arguments of one launch over the solver's buffers (also used by pools).
+void fill_args(const bex_solver* s, BexKernelArgs* ap) {
+  BexKernelArgs& a = *ap;
+  const int ND = s->d.UD;
+  a.NS = s->d.NS;
+  a.ActiveSystems = s->ActiveSystems;
+  a.NDO = s->d.NDO;
+  a.first_system = 0;
+  a.TLT = s->TLT;
+  a.TimeDomain = (double*)s->TimeDomain.d;
+  a.ActualState = (double*)s->ActualState.d;
+  a.ActualTime = (double*)s->ActualTime.d;
+  a.UnitParameters = (double*)s->UnitParameters.d;
+  a.SystemParameters = (double*)s->SystemParameters.d;
+  a.GlobalParameters = (double*)s->GlobalParameters.d;
+  a.IntegerGlobalParameters = (int*)s->IntegerGlobalParameters.d;
+  a.UnitAccessories = (double*)s->UnitAccessories.d;
+  a.IntegerUnitAccessories = (int*)s->IntegerUnitAccessories.d;
+  a.SystemAccessories = (double*)s->SystemAccessories.d;
+  a.IntegerSystemAccessories = (int*)s->IntegerSystemAccessories.d;
+  a.CouplingMatrix = s->PerSystemCoupling ? s->d_CouplingSys : s->d_CouplingFull;
+  a.CouplingMatrixT = s->PerSystemCoupling ? nullptr : s->d_CouplingFullT;
+  a.DenseOutputIndex = (int*)s->DenseOutputIndex.d;
+  a.DenseOutputTimeInstances = (double*)s->DenseOutputTimeInstances.d;
+  a.DenseOutputStates = (double*)s->DenseOutputStates.d;
+  a.RelativeTolerance = s->d_tol;
+  a.AbsoluteTolerance = s->d_tol + ND;
+  a.EventTolerance = s->d_tol + 2 * ND;
+  a.EventDirection = s->d_dir;
+  a.Stats = s->d_stats;
+  a.Timeline = s->d_timeline;
+  a.InitialTimeStep = s->InitialTimeStep;
+  a.MaximumTimeStep = s->MaximumTimeStep;
+  a.MinimumTimeStep = s->MinimumTimeStep;
+  a.TimeStepGrowLimit = s->TimeStepGrowLimit;
+  a.TimeStepShrinkLimit = s->TimeStepShrinkLimit;
+  a.DenseOutputMinimumTimeStep = s->DenseOutputMinimumTimeStep;
+  a.DenseOutputSaveFrequency = s->DenseOutputSaveFrequency;
+  a.BiCGRhoTolerance = s->BiCGRhoTolerance;
+  a.BiCGResidualTolerance = s->BiCGResidualTolerance;
+  a.BiCGMaxIterations = s->BiCGMaxIterations;
+  a.PriorityStepThreshold = s->PriorityStepThreshold > 0 ? s->PriorityStepThreshold : 0x3fffffff;
+  a.WarmStart = s->d_net;
+  a.Samples = s->d_samples;
+  a.SampleCount = s->d_sample_count;
+  a.SampleCapacity = s->SampleCapacity;
+  a.SampleEvery = s->SampleEvery > 0 ? s->SampleEvery : 1;
+  a.ref_spb = s->d.SPB;
+  a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
 }
 
 // Logical thread index of (sid, uid) in the reference layout (Interface.cuh:875-879).
@@ -309,7 +366,7 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   KernelChoice kc;
   if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
-                "bex_create: units per system must be 8, 16, 32 or 64 in this build");
+                "bex_create: units per system must be 8, 16, 32, 64 or 128 in this build");
 
   bex_solver* s = new bex_solver();
   s->d = d;
@@ -438,6 +495,7 @@ int bex_destroy(bex_solver* s) {
   if (s->h_CouplingOriginal) hipHostFree(s->h_CouplingOriginal);
   if (s->d_CouplingFull) hipFree(s->d_CouplingFull);
   if (s->d_CouplingFullT) hipFree(s->d_CouplingFullT);
+  if (s->d_CouplingSys) hipFree(s->d_CouplingSys);
   if (s->h_tol) hipHostFree(s->h_tol);
   if (s->h_dir) hipHostFree(s->h_dir);
   if (s->d_tol) hipFree(s->d_tol);
@@ -457,6 +515,7 @@ int bex_destroy(bex_solver* s) {
 
 int bex_solver_option(bex_solver* s, int option, double v) {
   if (!s) return BEX_E_ARGUMENT;
+  s->config_epoch++;
   switch (option) {
     case BEX_InitialTimeStep: s->InitialTimeStep = v; break;
     case BEX_ActiveSystems: s->ActiveSystems = (int)v; break;
@@ -496,6 +555,7 @@ int bex_solver_option(bex_solver* s, int option, double v) {
 
 int bex_solver_option_indexed(bex_solver* s, int option, int index, double v) {
   if (!s) return BEX_E_ARGUMENT;
+  s->config_epoch++;
   const int UD = s->d.UD, NE = s->d.NE;
   int rc;
   switch (option) {
@@ -787,57 +847,15 @@ int bex_solve(bex_solver* s) {
     return fail(s, BEX_E_ARGUMENT, "Solve: warm start enabled but no network was set");
   if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, &kc))
     return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count, layout and mode");
-  const int ND = s->d.UD;
   BexKernelArgs a{};
-  a.NS = s->d.NS;
-  a.ActiveSystems = s->ActiveSystems;
-  a.NDO = s->d.NDO;
-  a.first_system = 0;
-  a.TLT = s->TLT;
-  a.TimeDomain = (double*)s->TimeDomain.d;
-  a.ActualState = (double*)s->ActualState.d;
-  a.ActualTime = (double*)s->ActualTime.d;
-  a.UnitParameters = (double*)s->UnitParameters.d;
-  a.SystemParameters = (double*)s->SystemParameters.d;
-  a.GlobalParameters = (double*)s->GlobalParameters.d;
-  a.IntegerGlobalParameters = (int*)s->IntegerGlobalParameters.d;
-  a.UnitAccessories = (double*)s->UnitAccessories.d;
-  a.IntegerUnitAccessories = (int*)s->IntegerUnitAccessories.d;
-  a.SystemAccessories = (double*)s->SystemAccessories.d;
-  a.IntegerSystemAccessories = (int*)s->IntegerSystemAccessories.d;
-  a.CouplingMatrix = s->d_CouplingFull;
-  a.CouplingMatrixT = s->d_CouplingFullT;
-  a.DenseOutputIndex = (int*)s->DenseOutputIndex.d;
-  a.DenseOutputTimeInstances = (double*)s->DenseOutputTimeInstances.d;
-  a.DenseOutputStates = (double*)s->DenseOutputStates.d;
-  a.RelativeTolerance = s->d_tol;
-  a.AbsoluteTolerance = s->d_tol + ND;
-  a.EventTolerance = s->d_tol + 2 * ND;
-  a.EventDirection = s->d_dir;
-  a.Stats = s->d_stats;
-  a.Timeline = s->d_timeline;
-  a.InitialTimeStep = s->InitialTimeStep;
-  a.MaximumTimeStep = s->MaximumTimeStep;
-  a.MinimumTimeStep = s->MinimumTimeStep;
-  a.TimeStepGrowLimit = s->TimeStepGrowLimit;
-  a.TimeStepShrinkLimit = s->TimeStepShrinkLimit;
-  a.DenseOutputMinimumTimeStep = s->DenseOutputMinimumTimeStep;
-  a.DenseOutputSaveFrequency = s->DenseOutputSaveFrequency;
-  a.BiCGRhoTolerance = s->BiCGRhoTolerance;
-  a.BiCGResidualTolerance = s->BiCGResidualTolerance;
-  a.BiCGMaxIterations = s->BiCGMaxIterations;
-  a.PriorityStepThreshold = s->PriorityStepThreshold > 0 ? s->PriorityStepThreshold : 0x3fffffff;
-  a.WarmStart = s->d_net;
-  a.Samples = s->d_samples;
-  a.SampleCount = s->d_sample_count;
-  a.SampleCapacity = s->SampleCapacity;
-  a.SampleEvery = s->SampleEvery > 0 ? s->SampleEvery : 1;
-  a.ref_spb = s->d.SPB;
-  a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
+  fill_args(s, &a);
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
   if (nsys <= 0) return BEX_OK;
+  if (kc.lds > 65536)
+    BEX_HIP(s, hipFuncSetAttribute(reinterpret_cast<const void*>(kc.fn),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kc.lds));
   dim3 grid((nsys + kc.systems_per_block - 1) / kc.systems_per_block), block(kc.block);
-  hipLaunchKernelGGL(kc.fn, grid, block, kc.lds, s->stream, a);
+  hipLaunchKernelGGL(kc.fn, grid, block, kc.lds, s->stream, a, (long long)nsys);
   BEX_HIP(s, hipGetLastError());
   return BEX_OK;
 }
@@ -891,6 +909,7 @@ int bex_timer_stop(bex_solver* s, double* elapsed_ms) {
 }
 
 int bex_enable_timeline(bex_solver* s, int on) {
+  if (s) s->config_epoch++;
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
   if (on && !s->d_timeline) {
@@ -1000,5 +1019,441 @@ int bex_get_samples(bex_solver* s, float* out, long long max_rows, long long* ro
   }
   return BEX_OK;
 }
+
+}  // extern "C"
+
+// =========================================================================================
+// Pools of uneven ensembles (no reference counterpart: the reference's per-block solver holds
+// one N and one shared coupling matrix per object). A pool holds size classes; class c is a
+// solver of NS[c] systems of N[c] units with one coupling matrix per system. Each class runs
+// as a persistent kernel whose waves take systems from a device work queue in a given order
+// (longest expected run first); one period of all classes is one hipGraph whose branches run
+// the classes concurrently, replayed once per period.
+
+namespace {
+
+using PoolFn = void (*)(BexKernelArgs, const int*, long long, long long, unsigned long long*);
+
+struct PoolVariant {
+  const char* name;
+  PoolFn fn;
+  size_t lds;
+  int block;
+  int items_per_block;
+  int systems_per_item;
+};
+
+template <class L>
+PoolVariant pool_variant(const char* name) {
+  return {name, (PoolFn)bexk::pool_kernel<L, bexk::kRKCK45>, L::lds_bytes(), L::kBlock,
+          L::kItemsPerBlock, L::kSystemsPerItem};
+}
+
+const std::vector<PoolVariant>& pool_variants(int n) {
+  static const std::vector<PoolVariant> v8 = {
+      pool_variant<bexk::GroupLayout<8, 1, true>>("group8 x1 wave"),
+      pool_variant<bexk::GroupLayout<8, 2, true>>("group8 x2 waves"),
+      pool_variant<bexk::GroupLayout<8, 4, true>>("group8 x4 waves")};
+  static const std::vector<PoolVariant> v16 = {
+      pool_variant<bexk::GroupLayout<16, 1, true>>("group16 x1 wave"),
+      pool_variant<bexk::GroupLayout<16, 2, true>>("group16 x2 waves"),
+      pool_variant<bexk::GroupLayout<16, 4, true>>("group16 x4 waves")};
+  static const std::vector<PoolVariant> v32 = {
+      pool_variant<bexk::RolePairLayout32<bexk::kPlain, true>>("rolepair32"),
+      pool_variant<bexk::GroupLayout<32, 1, true>>("group32 x1 wave"),
+      pool_variant<bexk::GroupLayout<32, 2, true>>("group32 x2 waves")};
+  static const std::vector<PoolVariant> v64 = {
+      pool_variant<bexk::GroupLayout<64, 1, true>>("group64 x1 wave"),
+      pool_variant<bexk::GroupLayout<64, 2, true>>("group64 x2 waves")};
+  static const std::vector<PoolVariant> v128 = {
+      pool_variant<bexk::WideLayout<128, true>>("wide128 x2 waves")};
+  static const std::vector<PoolVariant> none;
+  switch (n) {
+    case 8: return v8;
+    case 16: return v16;
+    case 32: return v32;
+    case 64: return v64;
+    case 128: return v128;
+    default: return none;
+  }
+}
+
+__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
+
+}  // namespace
+
+struct bex_pool_class {
+  bex_solver* s = nullptr;
+  int N = 0, NS = 0;
+  int variant = 0;
+  int* d_order = nullptr;
+  unsigned long long* d_counter = nullptr;
+  int config_epoch = -1;  // class solver's configuration at graph capture
+};
+
+struct bex_pool {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<hipStream_t> side;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join, ready;
+  std::vector<bex_pool_class> cls;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+  int cu_count = 0;
+  std::string err;
+};
+
+namespace {
+
+int pfail(bex_pool* p, int code, const std::string& msg) {
+  if (p)
+    p->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define BEX_PHIP(p, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return pfail((p), BEX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Persistent launch of class c on stream st (queue counter cleared by a kernel first, so the
+// sequence can be captured into a graph).
+int launch_class(bex_pool* p, int c, hipStream_t st) {
+  bex_pool_class& k = p->cls[c];
+  const PoolVariant& v = pool_variants(k.N)[k.variant];
+  BexKernelArgs a{};
+  fill_args(k.s, &a);
+  const int nsys = k.s->ActiveSystems < k.NS ? k.s->ActiveSystems : k.NS;
+  if (nsys <= 0) return BEX_OK;
+  const long long nitems = (k.NS + v.systems_per_item - 1) / v.systems_per_item;
+  int per_cu = 0;
+  BEX_PHIP(p, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(v.fn),
+                                                           v.block, v.lds));
+  if (per_cu < 1) return pfail(p, BEX_E_UNSUPPORTED, "pool: a class kernel does not fit on a CU");
+  long long blocks = (long long)p->cu_count * per_cu;
+  const long long need = (nitems + v.items_per_block - 1) / v.items_per_block;
+  if (blocks > need) blocks = need;
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, k.d_counter);
+  hipLaunchKernelGGL(v.fn, dim3((unsigned)blocks), dim3(v.block), v.lds, st, a,
+                     (const int*)k.d_order, (long long)k.NS, nitems, k.d_counter);
+  BEX_PHIP(p, hipGetLastError());
+  return BEX_OK;
+}
+
+bool graph_stale(const bex_pool* p) {
+  if (!p->graph_valid) return true;
+  for (const auto& k : p->cls)
+    if (k.config_epoch != k.s->config_epoch) return true;
+  return false;
+}
+
+int capture(bex_pool* p) {
+  if (p->exec) {
+    hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  BEX_PHIP(p, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  int rc = BEX_OK;
+  if (hipEventRecord(p->fork, p->stream) != hipSuccess) rc = BEX_E_HIP;
+  for (size_t c = 0; rc == BEX_OK && c < p->cls.size(); c++) {
+    if (hipStreamWaitEvent(p->side[c], p->fork, 0) != hipSuccess) {
+      rc = BEX_E_HIP;
+      break;
+    }
+    rc = launch_class(p, (int)c, p->side[c]);
+    if (rc == BEX_OK && (hipEventRecord(p->join[c], p->side[c]) != hipSuccess ||
+                         hipStreamWaitEvent(p->stream, p->join[c], 0) != hipSuccess))
+      rc = BEX_E_HIP;
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (rc != BEX_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc == BEX_E_HIP ? pfail(p, rc, "pool: graph capture failed") : rc;
+  }
+  if (e != hipSuccess) return pfail(p, BEX_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  p->graph = g;
+  BEX_PHIP(p, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  for (auto& k : p->cls) k.config_epoch = k.s->config_epoch;
+  p->graph_valid = true;
+  return BEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bex_pool_variant_count(int units) { return (int)pool_variants(units).size(); }
+
+const char* bex_pool_variant_name(int units, int variant) {
+  const auto& v = pool_variants(units);
+  return (variant >= 0 && variant < (int)v.size()) ? v[variant].name : nullptr;
+}
+
+int bex_pool_create(int nclass, const int* units, const int* systems, int device, bex_pool** out) {
+  if (!out || !units || !systems || nclass <= 0) return BEX_E_ARGUMENT;
+  *out = nullptr;
+  bex_pool* p = new bex_pool();
+  p->device = device;
+  auto bail = [&](int code, const std::string& msg) {
+    bex_pool_destroy(p);
+    return pfail(nullptr, code, msg);
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(BEX_E_HIP, "bex_pool_create: hipSetDevice failed");
+  if (hipDeviceGetAttribute(&p->cu_count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return bail(BEX_E_HIP, "bex_pool_create: device query failed");
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess)
+    return bail(BEX_E_HIP, "bex_pool_create: stream/event creation failed");
+  bex_problem_desc dims{};
+  bex_compiled_dims(&dims);
+  for (int c = 0; c < nclass; c++) {
+    const int N = units[c], NS = systems[c];
+    if (pool_variants(N).empty() || NS <= 0)
+      return bail(BEX_E_UNSUPPORTED, "bex_pool_create: units per system must be 8, 16, 32, 64 or 128");
+    bex_problem_desc d = dims;
+    d.NS = NS;
+    d.UPS = N;
+    d.TPB = N;
+    d.SPB = 1;
+    d.NC = 1;
+    d.CBW = 0;
+    d.CCI = 0;
+    d.NDO = 0;
+    d.Algorithm = BEX_RKCK45;
+    bex_pool_class k;
+    k.N = N;
+    k.NS = NS;
+    p->cls.push_back(k);
+    bex_pool_class& kk = p->cls.back();
+    const int rc = bex_create(&d, device, &kk.s);
+    if (rc != BEX_OK) return bail(rc, g_create_error);
+    kk.s->PerSystemCoupling = true;
+    std::vector<int> order(NS);
+    for (int i = 0; i < NS; i++) order[i] = i;
+    if (hipMalloc((void**)&kk.s->d_CouplingSys, sizeof(double) * (size_t)NS * N * N) != hipSuccess ||
+        hipMemset(kk.s->d_CouplingSys, 0, sizeof(double) * (size_t)NS * N * N) != hipSuccess ||
+        hipMalloc((void**)&kk.d_order, sizeof(int) * (size_t)NS) != hipSuccess ||
+        hipMemcpy(kk.d_order, order.data(), sizeof(int) * (size_t)NS, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void**)&kk.d_counter, sizeof(unsigned long long)) != hipSuccess)
+      return bail(BEX_E_MEMORY, "bex_pool_create: not enough device memory");
+    hipStream_t st = nullptr;
+    hipEvent_t j = nullptr, r = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r, hipEventDisableTiming) != hipSuccess)
+      return bail(BEX_E_HIP, "bex_pool_create: stream/event creation failed");
+    p->side.push_back(st);
+    p->join.push_back(j);
+    p->ready.push_back(r);
+    for (const auto& v : pool_variants(N))
+      if (v.lds > 65536 &&
+          hipFuncSetAttribute(reinterpret_cast<const void*>(v.fn),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds) != hipSuccess)
+        return bail(BEX_E_HIP, "bex_pool_create: LDS attribute failed");
+  }
+  *out = p;
+  return BEX_OK;
+}
+
+int bex_pool_destroy(bex_pool* p) {
+  if (!p) return BEX_E_ARGUMENT;
+  hipSetDevice(p->device);
+  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  for (auto& k : p->cls) {
+    if (k.d_order) hipFree(k.d_order);
+    if (k.d_counter) hipFree(k.d_counter);
+    if (k.s) bex_destroy(k.s);
+  }
+  for (auto st : p->side) hipStreamDestroy(st);
+  for (auto e : p->join) hipEventDestroy(e);
+  for (auto e : p->ready) hipEventDestroy(e);
+  if (p->fork) hipEventDestroy(p->fork);
+  if (p->done) hipEventDestroy(p->done);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+  return BEX_OK;
+}
+
+int bex_pool_class_solver(bex_pool* p, int c, bex_solver** out) {
+  if (!p || !out || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  *out = p->cls[c].s;
+  return BEX_OK;
+}
+
+int bex_pool_set_couplings(bex_pool* p, int c, int first, int count, const double* M) {
+  if (!p || !M || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  bex_pool_class& k = p->cls[c];
+  if (first < 0 || count < 0 || first + count > k.NS)
+    return pfail(p, BEX_E_BOUND, "bex_pool_set_couplings: system range out of bounds");
+  BEX_PHIP(p, hipSetDevice(p->device));
+  BEX_PHIP(p, hipStreamSynchronize(k.s->stream));
+  const size_t nn = (size_t)k.N * k.N;
+  BEX_PHIP(p, hipMemcpy(k.s->d_CouplingSys + (size_t)first * nn, M, sizeof(double) * nn * count,
+                        hipMemcpyHostToDevice));
+  return BEX_OK;
+}
+
+int bex_pool_set_order(bex_pool* p, int c, const int* order) {
+  if (!p || !order || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  bex_pool_class& k = p->cls[c];
+  std::vector<char> seen(k.NS, 0);
+  for (int i = 0; i < k.NS; i++) {
+    if (order[i] < 0 || order[i] >= k.NS || seen[order[i]])
+      return pfail(p, BEX_E_ARGUMENT, "bex_pool_set_order: not a permutation of the class's systems");
+    seen[order[i]] = 1;
+  }
+  BEX_PHIP(p, hipSetDevice(p->device));
+  BEX_PHIP(p, hipStreamSynchronize(p->stream));
+  BEX_PHIP(p, hipMemcpy(k.d_order, order, sizeof(int) * (size_t)k.NS, hipMemcpyHostToDevice));
+  return BEX_OK;
+}
+
+// Orders class c's systems by the BiCG iterations they needed so far (their accumulated
+// counters), most expensive first; ties keep the index order.
+int bex_pool_order_by_cost(bex_pool* p, int c) {
+  if (!p || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  bex_pool_class& k = p->cls[c];
+  std::vector<long long> st((size_t)k.NS * BEX_NSTAT);
+  const int rc = bex_get_stats(k.s, st.data());
+  if (rc != BEX_OK) return pfail(p, rc, k.s->err);
+  std::vector<int> order(k.NS);
+  for (int i = 0; i < k.NS; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const long long cx = st[(size_t)x * BEX_NSTAT + BEX_STAT_BICG_ITERATIONS] +
+                         st[(size_t)x * BEX_NSTAT + BEX_STAT_STAGES];
+    const long long cy = st[(size_t)y * BEX_NSTAT + BEX_STAT_BICG_ITERATIONS] +
+                         st[(size_t)y * BEX_NSTAT + BEX_STAT_STAGES];
+    return cx > cy;
+  });
+  return bex_pool_set_order(p, c, order.data());
+}
+
+int bex_pool_set_variant(bex_pool* p, int c, int variant) {
+  if (!p || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  if (variant < 0 || variant >= (int)pool_variants(p->cls[c].N).size())
+    return pfail(p, BEX_E_ARGUMENT, "bex_pool_set_variant: no such variant");
+  p->cls[c].variant = variant;
+  p->graph_valid = false;
+  return BEX_OK;
+}
+
+int bex_pool_get_variant(bex_pool* p, int c, int* variant) {
+  if (!p || !variant || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  *variant = p->cls[c].variant;
+  return BEX_OK;
+}
+
+// One period (Solve) of every class: one graph launch, ordered after pending work of the class
+// solvers' streams and before their later work.
+int bex_pool_solve_periods(bex_pool* p, int periods) {
+  if (!p || periods < 0) return BEX_E_ARGUMENT;
+  BEX_PHIP(p, hipSetDevice(p->device));
+  if (graph_stale(p)) {
+    const int rc = capture(p);
+    if (rc != BEX_OK) return rc;
+  }
+  for (size_t c = 0; c < p->cls.size(); c++) {
+    BEX_PHIP(p, hipEventRecord(p->ready[c], p->cls[c].s->stream));
+    BEX_PHIP(p, hipStreamWaitEvent(p->stream, p->ready[c], 0));
+  }
+  for (int k = 0; k < periods; k++) BEX_PHIP(p, hipGraphLaunch(p->exec, p->stream));
+  BEX_PHIP(p, hipEventRecord(p->done, p->stream));
+  for (auto& k : p->cls) BEX_PHIP(p, hipStreamWaitEvent(k.s->stream, p->done, 0));
+  return BEX_OK;
+}
+
+int bex_pool_solve(bex_pool* p) { return bex_pool_solve_periods(p, 1); }
+
+int bex_pool_synchronise(bex_pool* p) {
+  if (!p) return BEX_E_ARGUMENT;
+  BEX_PHIP(p, hipSetDevice(p->device));
+  BEX_PHIP(p, hipStreamSynchronize(p->stream));
+  for (auto& k : p->cls) BEX_PHIP(p, hipStreamSynchronize(k.s->stream));
+  return BEX_OK;
+}
+
+// Times every launch variant of every class on the first `probe_fraction` of the current time
+// domain and keeps the fastest; device state, counters and the time domain are restored.
+int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
+  if (!p || !(probe_fraction > 0.0) || probe_fraction > 1.0) return BEX_E_ARGUMENT;
+  BEX_PHIP(p, hipSetDevice(p->device));
+  BEX_PHIP(p, hipDeviceSynchronize());
+  int slot = 0;
+  for (size_t c = 0; c < p->cls.size(); c++) {
+    bex_pool_class& k = p->cls[c];
+    bex_solver* s = k.s;
+    Buf* keep[] = {&s->ActualState, &s->ActualTime, &s->UnitAccessories, &s->IntegerUnitAccessories,
+                   &s->SystemAccessories, &s->IntegerSystemAccessories, &s->TimeDomain};
+    std::vector<void*> saved;
+    for (Buf* b : keep) {
+      void* d = nullptr;
+      BEX_PHIP(p, hipMalloc(&d, b->bytes()));
+      BEX_PHIP(p, hipMemcpy(d, b->d, b->bytes(), hipMemcpyDeviceToDevice));
+      saved.push_back(d);
+    }
+    long long* stats_saved = nullptr;
+    const size_t sbytes = sizeof(long long) * (size_t)k.NS * BEX_NSTAT;
+    BEX_PHIP(p, hipMalloc((void**)&stats_saved, sbytes));
+    BEX_PHIP(p, hipMemcpy(stats_saved, s->d_stats, sbytes, hipMemcpyDeviceToDevice));
+    // Probe time domain: [TD0, TD0 + f (TD1 - TD0)] from the device copy.
+    std::vector<double> td(2 * (size_t)k.NS);
+    BEX_PHIP(p, hipMemcpy(td.data(), s->TimeDomain.d, sizeof(double) * td.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < k.NS; i++) td[k.NS + i] = td[i] + probe_fraction * (td[k.NS + i] - td[i]);
+    hipEvent_t t0, t1;
+    BEX_PHIP(p, hipEventCreate(&t0));
+    BEX_PHIP(p, hipEventCreate(&t1));
+    const int nv = (int)pool_variants(k.N).size();
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int v = 0; v < nv; v++) {
+      for (size_t b = 0; b < saved.size(); b++)
+        BEX_PHIP(p, hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice));
+      BEX_PHIP(p, hipMemcpy(s->TimeDomain.d, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
+      const int old = k.variant;
+      k.variant = v;
+      BEX_PHIP(p, hipEventRecord(t0, p->stream));
+      const int rc = launch_class(p, (int)c, p->stream);
+      BEX_PHIP(p, hipEventRecord(t1, p->stream));
+      BEX_PHIP(p, hipEventSynchronize(t1));
+      k.variant = old;
+      if (rc != BEX_OK) return rc;
+      float ms = 0.0f;
+      BEX_PHIP(p, hipEventElapsedTime(&ms, t0, t1));
+      if (times_ms) times_ms[slot + v] = ms;
+      if (v == 0 || ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    slot += nv;
+    for (size_t b = 0; b < saved.size(); b++) {
+      BEX_PHIP(p, hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice));
+      hipFree(saved[b]);
+    }
+    BEX_PHIP(p, hipMemcpy(s->d_stats, stats_saved, sbytes, hipMemcpyDeviceToDevice));
+    hipFree(stats_saved);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    k.variant = best;
+  }
+  p->graph_valid = false;
+  return BEX_OK;
+}
+
+const char* bex_pool_last_error(const bex_pool* p) { return p ? p->err.c_str() : g_create_error.c_str(); }
 
 }  // extern "C"

@@ -93,27 +93,129 @@ struct BicgOut {
 };
 
 // =========================================================================================
-// GroupLayout<N>: lane = unit, 64/N systems per wave, 4 waves per workgroup.
+// Lane-per-unit layouts: GroupLayout<N, W, PS> (N <= 64, 64/N systems per wave, W waves per
+// workgroup) and WideLayout<N> (N = 128, two waves per system). PS = each system has its own
+// coupling matrix (pools of uneven ensembles); otherwise all systems share coupling 0.
+//
+// The coupling matrix lives in LDS. Shared matrices keep M and M^T column-major with stride N;
+// a per-system matrix keeps one copy, column-major with column stride N + 1, which serves both
+// products without bank conflicts: (M v)_i reads M[i + j (N+1)], (M^T v)_i reads M[j + i (N+1)].
+// Either way every row product is the canonical four-partial-sum order.
 
+// A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
+// (Steppers.cuh:489-561), for the lane-per-unit layouts. Returns x in F[1].
+template <class L>
+__device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, double x1, double cpt,
+                                                  double rD, const BexKernelArgs& a) {
+  const double x1sq = x1 * x1;
+  const double xu = F[1];
+  double2* vec = lay.vec;
+  const int u = lay.uid();
+  lay.sync();
+  vec[u].x = cpt;
+  lay.sync();
+  const double b = fma(rD, lay.mv(), xu);
+  lay.sync();
+  vec[u].x = x1sq * xu;
+  lay.sync();
+  const double ax = fma(rD, lay.mv(), xu);
+  double r = -ax + b;
+  double rt = r;
+  double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
+  double rho1 = lay.sum(r * rt);
+  BicgOut out{0, false, true, 0.0};
+  for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
+    if (iter == 1) {
+      p = r;
+      pt = rt;
+      if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
+        out.rho_exit = true;
+        out.maxit = false;
+        break;
+      }
+    } else {
+      const double beta = rho1 / rho2;
+      p = r + beta * p;
+      pt = rt + beta * pt;
+    }
+    lay.sync();
+    vec[u] = make_double2(x1sq * p, rD * pt);
+    lay.sync();
+    const double a1 = lay.mv();
+    const double a2 = lay.mtv();
+    const double q = fma(rD, a1, p);
+    const double qt = fma(x1sq, a2, pt);
+    out.iterations++;
+    const double alpha = rho1 / lay.sum(pt * q);
+    x = x + alpha * p;
+    r = r - alpha * q;
+    rt = rt - alpha * qt;
+    rho2 = rho1;
+    const double rmax = lay.fmax_all(bex::fabs_d(r));
+    rho1 = lay.sum(r * rt);
+    if (rmax < a.BiCGResidualTolerance) {
+      out.maxit = false;
+      break;
+    }
+  }
+  F[1] = x;
+  return out;
+}
+
+// Row products in the canonical order: four partial sums over columns j = q (mod 4), each a
+// sequential fma chain, combined as (s0 + s1) + (s2 + s3). m(j) is the row's j-th element,
+// z(j) the vector's.
+template <int N, class MF, class ZF>
+__device__ __forceinline__ double row_dot4(MF m, ZF z) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+  for (int j = 0; j < N; j += 4) {
+    s0 = fma(m(j), z(j), s0);
+    s1 = fma(m(j + 1), z(j + 1), s1);
+    s2 = fma(m(j + 2), z(j + 2), s2);
+    s3 = fma(m(j + 3), z(j + 3), s3);
+  }
+  return (s0 + s1) + (s2 + s3);
+}
+
+// Copies system sid's coupling matrix (column-major, stride N, in global memory) into a padded
+// LDS copy (column stride N + 1); lanes u = 0..N-1 of the system's group cooperate.
 template <int N>
+__device__ __forceinline__ void load_padded(double* dst, const double* src, int u, int nthreads) {
+  for (int k = u; k < N * N; k += nthreads) {
+    const int row = k % N, col = k / N;
+    dst[row + col * (N + 1)] = src[k];
+  }
+}
+
+template <int N, int W = 4, bool PS = false>
 struct GroupLayout {
   static_assert(N == 8 || N == 16 || N == 32 || N == 64, "units per system: 8, 16, 32 or 64");
+  static_assert(W >= 1 && W <= 4, "1 to 4 waves per workgroup");
   static constexpr int kUnits = N;
   static constexpr int kSystemsPerWave = kWave / N;
-  static constexpr int kWaves = 4;
+  static constexpr int kWaves = W;
   static constexpr int kBlock = kWave * kWaves;
   static constexpr int kSystemsPerBlock = kSystemsPerWave * kWaves;
   static constexpr int kWavesPerEU = 4;  // <= 128 VGPRs
   static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = W;  // persistent kernels: each wave pops its own item
+  static constexpr int kSystemsPerItem = kSystemsPerWave;
+  static constexpr int kLd = PS ? N + 1 : N;
 
+  static constexpr size_t m_bytes() {
+    return PS ? sizeof(double) * kSystemsPerBlock * N * (N + 1) : sizeof(double) * 2 * N * N;
+  }
   static constexpr size_t lds_bytes() {
-    return sizeof(double) * 2 * N * N + sizeof(double2) * kSystemsPerBlock * N +
-           sizeof(Sys) * kSystemsPerBlock + sizeof(int) * kSystemsPerBlock * BEX_NSTAT;
+    return m_bytes() + sizeof(double2) * kSystemsPerBlock * N + sizeof(Sys) * kSystemsPerBlock +
+           sizeof(int) * kSystemsPerBlock * BEX_NSTAT;
   }
 
-  int lane, grp, uid_, lsys;
-  const double* sM;
-  const double* sMT;
+  int lane, grp, uid_, lsys, wave, sid_;
+  const double* sM;   // shared: M; per-system: this system's padded copy
+  const double* sMT;  // shared: M^T
+  double* myM;
   double2* vec;
   Sys* sys;
   int* stat;
@@ -121,32 +223,57 @@ struct GroupLayout {
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     const int tid = threadIdx.x;
     lane = tid & (kWave - 1);
+    wave = tid / kWave;
     grp = lane / N;
     uid_ = lane % N;
-    lsys = (tid / kWave) * kSystemsPerWave + grp;
+    lsys = wave * kSystemsPerWave + grp;
     double* m = reinterpret_cast<double*>(smem);
-    double* mt = m + N * N;
-    double2* v = reinterpret_cast<double2*>(mt + N * N);
+    double2* v = reinterpret_cast<double2*>(smem + m_bytes());
     Sys* s = reinterpret_cast<Sys*>(v + kSystemsPerBlock * N);
     int* st = reinterpret_cast<int*>(s + kSystemsPerBlock);
-    for (int k = tid; k < N * N; k += kBlock) {
-      m[k] = a.CouplingMatrix[k];
-      mt[k] = a.CouplingMatrixT[k];
+    if (PS) {
+      myM = m + (size_t)lsys * N * (N + 1);
+      sM = myM;
+      sMT = myM;
+    } else {
+      double* mt = m + N * N;
+      for (int k = tid; k < N * N; k += kBlock) {
+        m[k] = a.CouplingMatrix[k];
+        mt[k] = a.CouplingMatrixT[k];
+      }
+      __syncthreads();  // the only workgroup barrier
+      sM = m;
+      sMT = mt;
+      myM = nullptr;
     }
-    for (int k = tid; k < kSystemsPerBlock * BEX_NSTAT; k += kBlock) st[k] = 0;
-    __syncthreads();  // the only workgroup barrier
-    sM = m;
-    sMT = mt;
     vec = v + lsys * N;
     sys = s + lsys;
     stat = st + lsys * BEX_NSTAT;
   }
 
-  __device__ __forceinline__ int system_in_launch() const {
-    return blockIdx.x * kSystemsPerBlock + lsys;
+  // Assigns this lane's system for work item `item` (one item = the systems of one wave):
+  // order == nullptr -> systems first_system + item * kSystemsPerWave + grp; otherwise the
+  // order array's entries. Loads a per-system matrix and clears the group's counters.
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    const long long pos = item * kSystemsPerWave + grp;
+    int s = -1;
+    if (order) {
+      if (pos < count) s = order[pos];
+    } else if (pos < count) {
+      s = a.first_system + (int)pos;
+    }
+    sid_ = s;
+    for (int k = uid_; k < BEX_NSTAT; k += N) stat[k] = 0;
+    if (PS && s >= 0 && s < a.NS)
+      load_padded<N>(myM, a.CouplingMatrix + (size_t)s * N * N, uid_, N);
+    wave_sync();
   }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
   __device__ __forceinline__ int uid() const { return uid_; }
   __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
 
   __device__ __forceinline__ unsigned long long mask() const {
     return (N == 64) ? ~0ull : (((1ull << N) - 1ull) << (grp * N));
@@ -176,78 +303,145 @@ struct GroupLayout {
     return v;
   }
 
-  // Row product in the canonical order: four partial sums over columns j = q (mod 4), each a
-  // sequential fma chain, combined as (s0 + s1) + (s2 + s3).
-  template <bool Y>
-  __device__ __forceinline__ double row_dot4(const double* mat) const {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll 4
-    for (int j = 0; j < N; j += 4) {
-      s0 = fma(mat[uid_ + j * N], Y ? vec[j].y : vec[j].x, s0);
-      s1 = fma(mat[uid_ + (j + 1) * N], Y ? vec[j + 1].y : vec[j + 1].x, s1);
-      s2 = fma(mat[uid_ + (j + 2) * N], Y ? vec[j + 2].y : vec[j + 2].x, s2);
-      s3 = fma(mat[uid_ + (j + 3) * N], Y ? vec[j + 3].y : vec[j + 3].x, s3);
+  __device__ __forceinline__ double mv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = uid_;
+    return row_dot4<N>([&](int j) { return m[i + j * kLd]; }, [&](int j) { return z[j].x; });
+  }
+  __device__ __forceinline__ double mtv() const {
+    const double2* z = vec;
+    const int i = uid_;
+    if (PS) {
+      const double* m = sM;
+      return row_dot4<N>([&](int j) { return m[j + i * kLd]; }, [&](int j) { return z[j].y; });
     }
-    return (s0 + s1) + (s2 + s3);
+    const double* mt = sMT;
+    return row_dot4<N>([&](int j) { return mt[i + j * N]; }, [&](int j) { return z[j].y; });
   }
 
-  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
-  // (Steppers.cuh:489-561). Returns x in F[1].
   template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
                                               const BexKernelArgs& a, Feat&&, bool) const {
-    const double x1sq = x1 * x1;
-    const double xu = F[1];
-    wave_sync();
-    vec[uid_].x = cpt;
-    wave_sync();
-    const double b = fma(rD, row_dot4<false>(sM), xu);
-    wave_sync();
-    vec[uid_].x = x1sq * xu;
-    wave_sync();
-    const double ax = fma(rD, row_dot4<false>(sM), xu);
-    double r = -ax + b;
-    double rt = r;
-    double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
-    double rho1 = sum(r * rt);
-    BicgOut out{0, false, true, 0.0};
-    for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
-      if (iter == 1) {
-        p = r;
-        pt = rt;
-        if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
-          out.rho_exit = true;
-          out.maxit = false;
-          break;
-        }
-      } else {
-        const double beta = rho1 / rho2;
-        p = r + beta * p;
-        pt = rt + beta * pt;
-      }
-      wave_sync();
-      vec[uid_] = make_double2(x1sq * p, rD * pt);
-      wave_sync();
-      const double a1 = row_dot4<false>(sM);
-      const double a2 = row_dot4<true>(sMT);
-      const double q = fma(rD, a1, p);
-      const double qt = fma(x1sq, a2, pt);
-      out.iterations++;
-      const double alpha = rho1 / sum(pt * q);
-      x = x + alpha * p;
-      r = r - alpha * q;
-      rt = rt - alpha * qt;
-      rho2 = rho1;
-      // The residual test and the next iteration's r.r~ are independent: evaluate together.
-      const double rmax = fmax_all(bex::fabs_d(r));
-      rho1 = sum(r * rt);
-      if (rmax < a.BiCGResidualTolerance) {
-        out.maxit = false;
-        break;
-      }
+    return group_coupling(*this, F, x1, cpt, rD, a);
+  }
+};
+
+// N = 128: one system per workgroup of two waves, lane = unit. Reductions combine the halves
+// through LDS (first tree level s[i] + s[i + 64]) and finish with an in-wave butterfly that
+// both waves evaluate identically; system-scope synchronisation is the workgroup barrier.
+template <int N, bool PS = true>
+struct WideLayout {
+  static_assert(N == 128, "two-wave layout: 128 units");
+  static constexpr int kUnits = N;
+  static constexpr int kWaves = N / kWave;
+  static constexpr int kBlock = N;
+  static constexpr int kSystemsPerBlock = 1;
+  static constexpr int kWavesPerEU = 4;
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 1;
+  static constexpr int kLd = N + 1;
+
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * N * (N + 1) + sizeof(double2) * N + sizeof(double) * N +
+           sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
+  }
+
+  int tid, sid_;
+  double* sM;
+  double2* vec;
+  double* red;
+  Sys* sys;
+  int* stat;
+  int* flags;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
+    tid = threadIdx.x;
+    sM = reinterpret_cast<double*>(smem);
+    vec = reinterpret_cast<double2*>(sM + N * (N + 1));
+    red = reinterpret_cast<double*>(vec + N);
+    sys = reinterpret_cast<Sys*>(red + N);
+    stat = reinterpret_cast<int*>(sys + 1);
+    flags = stat + BEX_NSTAT;
+    if (!PS) {
+      load_padded<N>(sM, a.CouplingMatrix, tid, kBlock);
+      __syncthreads();
     }
-    F[1] = x;
-    return out;
+  }
+
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    int s = -1;
+    if (item < count) s = order ? order[item] : a.first_system + (int)item;
+    sid_ = s;
+    if (tid < BEX_NSTAT) stat[tid] = 0;
+    if (PS && s >= 0 && s < a.NS) load_padded<N>(sM, a.CouplingMatrix + (size_t)s * N * N, tid, kBlock);
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return tid; }
+  __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+
+  template <class Op>
+  __device__ __forceinline__ double reduce(double v, Op op) const {
+    red[tid] = v;
+    __syncthreads();
+    const int l = tid & (kWave - 1);
+    double w = op(red[l], red[l + kWave]);
+#pragma unroll
+    for (int h = kWave / 2; h >= 1; h >>= 1) w = op(w, __shfl_xor(w, h, kWave));
+    __syncthreads();
+    return w;
+  }
+  __device__ __forceinline__ double sum(double v) const {
+    return reduce(v, [](double x, double y) { return x + y; });
+  }
+  __device__ __forceinline__ double fmax_all(double v) const {
+    return reduce(v, [](double x, double y) { return bex::fmax_d(x, y); });
+  }
+  __device__ __forceinline__ double min_keep(double v) const {
+    return reduce(v, [](double x, double y) { return (y < x) ? y : x; });
+  }
+  __device__ __forceinline__ bool all(bool p) const {
+    if (tid == 0) flags[0] = 0;
+    __syncthreads();
+    if (!p) flags[0] = 1;
+    __syncthreads();
+    const bool r = flags[0] == 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool any(bool p) const {
+    if (tid == 0) flags[1] = 0;
+    __syncthreads();
+    if (p) flags[1] = 1;
+    __syncthreads();
+    const bool r = flags[1] != 0;
+    __syncthreads();
+    return r;
+  }
+
+  __device__ __forceinline__ double mv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = tid;
+    return row_dot4<N>([&](int j) { return m[i + j * kLd]; }, [&](int j) { return z[j].x; });
+  }
+  __device__ __forceinline__ double mtv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = tid;
+    return row_dot4<N>([&](int j) { return m[j + i * kLd]; }, [&](int j) { return z[j].y; });
+  }
+
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    return group_coupling(*this, F, x1, cpt, rD, a);
   }
 };
 
@@ -403,7 +597,7 @@ __device__ __forceinline__ float mlp_factor(const BexMlpDevice* net, const doubl
   return net->b3 + (p0 + p1);
 }
 
-template <int MODE>
+template <int MODE, bool PS = false>
 struct RolePairLayout32 {
   static constexpr int kUnits = 32;
   static constexpr int kSystemsPerWave = 1;
@@ -413,12 +607,15 @@ struct RolePairLayout32 {
   // The learned kernel needs ~70 more registers for the MFMA tiles: 2 waves per SIMD.
   static constexpr int kWavesPerEU = (MODE == kLearned) ? 2 : BEX_ROLEPAIR_WAVES_PER_EU;
   static constexpr int kMode = MODE;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 1;
 
   static constexpr size_t lds_bytes() {
     return (MODE == kLearned ? sizeof(BexMlpDevice) : 0) + sizeof(Sys) + sizeof(int) * BEX_NSTAT;
   }
 
-  int lane;
+  int lane, sid_;
   bool primal_;
   double mrow[32];  // row i of M (primal) or of M^T (shadow)
   Sys* sys;
@@ -428,10 +625,6 @@ struct RolePairLayout32 {
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     lane = threadIdx.x;
     primal_ = lane < 32;
-    const int i = lane & 31;
-    const double* src = primal_ ? a.CouplingMatrix : a.CouplingMatrixT;
-#pragma unroll
-    for (int j = 0; j < 32; j++) mrow[j] = src[i + j * 32];
     unsigned char* p = smem;
     if (MODE == kLearned) {
       BexMlpDevice* n = reinterpret_cast<BexMlpDevice*>(p);
@@ -443,13 +636,40 @@ struct RolePairLayout32 {
     }
     sys = reinterpret_cast<Sys*>(p);
     stat = reinterpret_cast<int*>(sys + 1);
+  }
+
+  // One work item = one system: first_system + item, or order[item]. Loads the lane's matrix
+  // row (shared coupling 0 and its transpose, or the system's own matrix) into registers.
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    int s = -1;
+    if (item < count) s = order ? order[item] : a.first_system + (int)item;
+    sid_ = s;
     if (lane < BEX_NSTAT) stat[lane] = 0;
+    const int i = lane & 31;
+    if (s >= 0 && s < a.NS) {
+      if (PS) {
+        const double* m = a.CouplingMatrix + (size_t)s * 1024;
+        if (primal_) {
+#pragma unroll
+          for (int j = 0; j < 32; j++) mrow[j] = m[i + j * 32];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 32; j++) mrow[j] = m[j + i * 32];
+        }
+      } else {
+        const double* src = primal_ ? a.CouplingMatrix : a.CouplingMatrixT;
+#pragma unroll
+        for (int j = 0; j < 32; j++) mrow[j] = src[i + j * 32];
+      }
+    }
     wave_sync();
   }
 
-  __device__ __forceinline__ int system_in_launch() const { return blockIdx.x; }
+  __device__ __forceinline__ int sid() const { return sid_; }
   __device__ __forceinline__ int uid() const { return lane & 31; }
   __device__ __forceinline__ bool primal() const { return primal_; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
 
   // Both halves hold identical per-unit values, so wave-wide AND/OR/min equal the system's.
   __device__ __forceinline__ bool all(bool p) const { return __ballot(!p) == 0ull; }
@@ -623,8 +843,8 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   constexpr int NE = D::NE;
   static_assert(UD == 2, "the implicit coupling acts on component 1 of a two-component unit");
 
-  const int sid = a.first_system + lay.system_in_launch();
-  const bool active = (sid < a.NS) && (sid < a.ActiveSystems);  // Solver.cuh:82,132
+  const int sid = lay.sid();
+  const bool active = (sid >= 0) && (sid < a.NS) && (sid < a.ActiveSystems);  // Solver.cuh:82,132
   if (!active) return;
 
   Sys& S = *lay.sys;
@@ -676,14 +896,14 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     etol[i] = a.EventTolerance[i];
     edir[i] = a.EventDirection[i];
   }
-  wave_sync();
+  lay.sync();
 
   // Initialization hook (Solver.cuh:262-281).
   CoupledSystems_PerBlock_Initialization<double>(sid, uid, S.DOIDX, S.T, S.dT, S.TD, x, upar,
                                                  S.sPAR, a.GlobalParameters,
                                                  a.IntegerGlobalParameters, uacc, iuacc, S.sACC,
                                                  S.isACC);
-  wave_sync();
+  lay.sync();
   double t = S.T;
   double dt = a.InitialTimeStep;
   double dtNew = a.InitialTimeStep;
@@ -707,9 +927,9 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
           a.DenseOutputStates[gtid + i * TLT + (long long)idx * TLT * UD] = x[i];
         if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = t;
       }
-      wave_sync();
+      lay.sync();
       if (uid == 0) S.DOIDX = idx + 1;
-      wave_sync();
+      lay.sync();
       skipped = 0;
       doTime = bex::fmin_d(t + a.DenseOutputMinimumTimeStep, TD1);
     }
@@ -932,11 +1152,11 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         S.T = t;
         S.dT = dt;
       }
-      wave_sync();
+      lay.sync();
       CoupledSystems_PerBlock_ActionAfterSuccessfulTimeStep<double>(
           sid, uid, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
           a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
-      wave_sync();
+      lay.sync();
       if (NE > 0) {
 #pragma unroll
         for (int i = 0; i < NE; i++) {
@@ -948,7 +1168,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
             if (primal) atomicAdd(&st[BEX_STAT_EVENTS], 1);
           }
         }
-        wave_sync();
+        lay.sync();
         CoupledSystems_PerBlock_EventFunction<double>(sid, uid, en, S.T, dtNew, S.TD, x, upar,
                                                       S.sPAR, a.GlobalParameters,
                                                       a.IntegerGlobalParameters, uacc, iuacc,
@@ -956,7 +1176,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
 #pragma unroll
         for (int i = 0; i < NE; i++) ea[i] = en[i];
       }
-      wave_sync();
+      lay.sync();
       t = S.T;
     } else if (leader) {
       st[BEX_STAT_REJECTED]++;
@@ -981,12 +1201,12 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
 
   // Finalization hook and write back (Solver.cuh:577-647).
   if (uid == 0) S.T = t;
-  wave_sync();
+  lay.sync();
   CoupledSystems_PerBlock_Finalization<double>(sid, uid, S.DOIDX, S.T, S.dT, S.TD, x, upar,
                                                S.sPAR, a.GlobalParameters,
                                                a.IntegerGlobalParameters, uacc, iuacc, S.sACC,
                                                S.isACC);
-  wave_sync();
+  lay.sync();
   if (primal) {
     // Recompute the write-back addresses here instead of keeping the load-time ones alive
     // across the time loop (they would occupy ~2 registers per accessory).
@@ -1007,7 +1227,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     for (int i = 0; i < D::NiSA; i++) a.IntegerSystemAccessories[sid + i * NS] = S.isACC[i];
     if (a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
   }
-  wave_sync();
+  lay.sync();
   if (primal)
     for (int k = uid; k < BEX_NSTAT; k += N) a.Stats[(long long)sid * BEX_NSTAT + k] += st[k];
   if (leader && a.Timeline) a.Timeline[2 * (long long)sid + 1] = (long long)wall_clock64();
@@ -1015,11 +1235,39 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
 
 template <class L, int ALG>
 __global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
-void solve_kernel(const BexKernelArgs a) {
+void solve_kernel(const BexKernelArgs a, long long count) {
   extern __shared__ __align__(16) unsigned char smem[];
   L lay;
   lay.init(smem, a);
+  lay.bind(a, nullptr, (long long)blockIdx.x * L::kItemsPerBlock + threadIdx.x / (L::kBlock / L::kItemsPerBlock),
+           count);
   integrate<L, ALG>(lay, a);
+}
+
+// Persistent variant: a fixed grid whose waves (or workgroups, for multi-wave systems) take
+// work items from a device queue until it is empty. order lists the systems in the order they
+// are handed out (longest expected run first); items are kSystemsPerItem consecutive entries.
+// *counter must be zero at launch; every wave exits once the counter passes nitems.
+template <class L, int ALG>
+__global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
+void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long count,
+                 long long nitems, unsigned long long* counter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  L lay;
+  lay.init(smem, a);
+  __shared__ unsigned long long taken[L::kItemsPerBlock];
+  const int slot = threadIdx.x / (L::kBlock / L::kItemsPerBlock);
+  const bool leader = (threadIdx.x % (L::kBlock / L::kItemsPerBlock)) == 0;
+  for (;;) {
+    if (leader) taken[slot] = atomicAdd(counter, 1ull);
+    lay.sync();
+    const long long item = (long long)taken[slot];
+    lay.sync();
+    if (item >= nitems) break;
+    lay.bind(a, order, item, count);
+    integrate<L, ALG>(lay, a);
+    lay.sync();
+  }
 }
 
 }  // namespace bexk

@@ -95,22 +95,27 @@ class ProblemSolver:
 
     def __init__(self, NS, UPS, UD=2, TPB=None, SPB=1, NC=1, CBW=0, CCI=0, NUP=21, NSP=0, NGP=0,
                  NiGP=0, NUA=64, NiUA=1, NSA=0, NiSA=0, NE=1, NDO=0,
-                 Algorithm=Algorithms.RKCK45, device=0):
+                 Algorithm=Algorithms.RKCK45, device=0, _handle=None):
         if TPB is None:
             TPB = SPB * UPS
         self.desc = _native.ProblemDesc(NS, UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP, NiGP,
                                         NUA, NiUA, NSA, NiSA, NE, NDO, int(Algorithm))
         self._lib = _native.load()
-        h = ctypes.c_void_p()
-        _native.check(self._lib.bex_create(ctypes.byref(self.desc), device, ctypes.byref(h)))
-        self._h = h
+        self._owned = _handle is None
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _native.check(self._lib.bex_create(ctypes.byref(self.desc), device, ctypes.byref(h)))
+            self._h = h
+        else:
+            self._h = _handle
         self.NS, self.UPS, self.UD, self.NDO = NS, UPS, UD, NDO
         self.dims = {f: getattr(self.desc, f) for f, _ in self.desc._fields_}
 
     # -- lifetime --------------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.bex_destroy(self._h)
+            if self._owned:
+                self._lib.bex_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -278,3 +283,106 @@ class ProblemSolver:
     def event_counts(self):
         v = self.host_view(ListOfVariables.IntegerUnitAccessories)
         return v.reshape(-1, self.NS, self.UPS)[0] + 1
+
+
+class ProblemPool:
+    """Uneven ensemble: size classes of systems with their own coupling matrices, solved by
+    persistent kernels with device work queues, one hipGraph launch per period (C ABI
+    bex_pool_*; no reference counterpart).
+
+    solver(c) returns class c's ProblemSolver (options, SetHost/GetHost, host views and
+    synchronisation work as for a stand-alone solver; its Solve() runs that class alone)."""
+
+    def __init__(self, units, systems, NUP=21, NUA=64, NiUA=1, NE=1, device=0):
+        self._lib = _native.load()
+        units = [int(u) for u in units]
+        systems = [int(n) for n in systems]
+        if len(units) != len(systems):
+            raise ValueError("units and systems must have the same length")
+        n = len(units)
+        h = ctypes.c_void_p()
+        _native.pool_check(self._lib.bex_pool_create(
+            n, (ctypes.c_int * n)(*units), (ctypes.c_int * n)(*systems), device, ctypes.byref(h)))
+        self._h = h
+        self.units, self.systems = units, systems
+        self._solvers = []
+        for c in range(n):
+            sh = ctypes.c_void_p()
+            _native.pool_check(self._lib.bex_pool_class_solver(self._h, c, ctypes.byref(sh)), self._h)
+            self._solvers.append(ProblemSolver(systems[c], units[c], NUP=NUP, NUA=NUA, NiUA=NiUA,
+                                               NE=NE, _handle=sh))
+
+    def _check(self, rc):
+        return _native.pool_check(rc, self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for s in self._solvers:
+                s.close()
+            self._lib.bex_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solver(self, c):
+        return self._solvers[c]
+
+    def set_couplings(self, c, M, first=0):
+        """M: [count, N, N] with M[s, i, j] the coupling of unit i to unit j of system
+        first + s."""
+        M = np.asarray(M, dtype=np.float64)
+        colmajor = np.ascontiguousarray(np.swapaxes(M, -1, -2))
+        self._check(self._lib.bex_pool_set_couplings(
+            self._h, c, first, M.shape[0], colmajor.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def set_order(self, c, order):
+        o = np.ascontiguousarray(np.asarray(order, dtype=np.int32))
+        self._check(self._lib.bex_pool_set_order(self._h, c,
+                                                 o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def order_by_cost(self, c):
+        self._check(self._lib.bex_pool_order_by_cost(self._h, c))
+
+    def Solve(self):
+        self._check(self._lib.bex_pool_solve(self._h))
+
+    def solve_periods(self, periods):
+        self._check(self._lib.bex_pool_solve_periods(self._h, int(periods)))
+
+    def synchronise(self):
+        self._check(self._lib.bex_pool_synchronise(self._h))
+
+    def variants(self, c):
+        n = self.units[c]
+        return [self._lib.bex_pool_variant_name(n, v).decode()
+                for v in range(self._lib.bex_pool_variant_count(n))]
+
+    def set_variant(self, c, v):
+        self._check(self._lib.bex_pool_set_variant(self._h, c, int(v)))
+
+    def variant(self, c):
+        v = ctypes.c_int()
+        self._check(self._lib.bex_pool_get_variant(self._h, c, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, probe_fraction=0.05):
+        """Returns {class: {variant name: probe ms}} and leaves the fastest variant selected."""
+        counts = [self._lib.bex_pool_variant_count(n) for n in self.units]
+        times = (ctypes.c_double * sum(counts))()
+        self._check(self._lib.bex_pool_autotune(self._h, float(probe_fraction), times))
+        out, k = {}, 0
+        for c, nv in enumerate(counts):
+            names = self.variants(c)
+            out[c] = {names[v]: times[k + v] for v in range(nv)}
+            k += nv
+        return out

@@ -219,3 +219,71 @@ def small_ensemble(n=8, ns=4, seed=7, pa=(0.8, 1.2), f=(40.0, 120.0), periods=2)
     pa_vals = np.linspace(pa[0], pa[1], max(1, int(round(math.sqrt(ns)))))
     f_vals = np.linspace(f[0], f[1], max(1, ns // max(1, len(pa_vals))))
     return _grid_ensemble("small", n, pa_vals, f_vals, seed, periods, False)
+
+
+def random_positions_batch(ns, n, min_sep_um, rng, box_um=None, sphere_radius_um=None,
+                           max_rounds=100000):
+    """Rejection-sampled 3D positions for ns independent systems at once: bubble k of every
+    system is redrawn until it keeps min_sep_um from bubbles 0..k-1 of its system.
+    Returns positions [ns, n, 3] and distances [ns, n, n]."""
+    pts = np.zeros((ns, n, 3))
+
+    def draw(m):
+        if sphere_radius_um is None:
+            return rng.uniform(0.0, box_um, size=(m, 3))
+        out = np.empty((m, 3))
+        todo = np.arange(m)
+        while todo.size:
+            c = rng.uniform(-sphere_radius_um, sphere_radius_um, size=(todo.size, 3))
+            ok = np.sum(c * c, axis=1) <= sphere_radius_um ** 2
+            out[todo[ok]] = c[ok]
+            todo = todo[~ok]
+        return out
+
+    for k in range(n):
+        todo = np.arange(ns)
+        rounds = 0
+        while todo.size:
+            rounds += 1
+            if rounds > max_rounds:
+                raise RuntimeError("could not place bubbles with the requested separation")
+            cand = draw(todo.size)
+            if k:
+                d2 = np.sum((pts[todo, :k, :] - cand[:, None, :]) ** 2, axis=-1)
+                ok = np.min(d2, axis=1) >= min_sep_um ** 2
+            else:
+                ok = np.ones(todo.size, dtype=bool)
+            pts[todo[ok], k] = cand[ok]
+            todo = todo[~ok]
+    d = np.sqrt(np.sum((pts[:, :, None, :] - pts[:, None, :, :]) ** 2, axis=-1))
+    return pts, d
+
+
+CONFIG2_UNITS = (8, 16, 32, 64, 128)
+
+
+def config2_cube_side_um(n, min_sep_um=50.0):
+    """Edge of the cube that holds an N-bubble system of config 2 (the configuration leaves it
+    open): six times the volume of N cubes of the minimum separation."""
+    return min_sep_um * (6.0 * n) ** (1.0 / 3.0)
+
+
+def config2_class(n, ns=4096, seed=2025, periods=64):
+    """One size class of config 2: ns systems of n bubbles at random positions in a cube with
+    50 um minimum separation, per-system coupling M_ij = R_j^3/(R_i^2 d_ij), radii log-uniform
+    1-10 um, forcing amplitude 0.3-2.5 bar and frequency 20-500 kHz drawn per system."""
+    rng = np.random.default_rng([seed, n])
+    _, d = random_positions_batch(ns, n, 50.0, rng, box_um=config2_cube_side_um(n))
+    re = np.exp(rng.uniform(math.log(1.0), math.log(10.0), size=(ns, n)))
+    pa = rng.uniform(0.3, 2.5, size=ns)
+    f = rng.uniform(20.0, 500.0, size=ns)
+    params = unit_parameters(pa[:, None], f[:, None], re)
+    M = coupling_from_geometry(re, d)
+    state0 = np.zeros((ns, n, 2))
+    state0[..., 0] = 1.0
+    return Ensemble(f"cfg2-N{n}", n, pa, f, re, params, M, state0, default_options(), periods)
+
+
+def config2(ns_per_class=4096, seed=2025, units=CONFIG2_UNITS):
+    """Config 2: 5 x 4096 systems with N in {8, 16, 32, 64, 128}, one Ensemble per size."""
+    return [config2_class(n, ns_per_class, seed) for n in units]

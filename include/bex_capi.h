@@ -199,6 +199,38 @@ int bex_mlp_predict(const float* rows, int stride, long long n, const float* par
 int bex_mlp_predict_mfma(const float* rows, int stride, long long n, const float* params,
                          float* out, void* stream);
 
+/* Pools of uneven ensembles (no reference counterpart). A pool holds `nclass` size classes;
+ * class c is a solver (bex_pool_class_solver: options, Set/Get, host views, synchronisation) of
+ * systems[c] systems of units[c] units (8, 16, 32, 64 or 128), RKCK45, each system with its own
+ * coupling matrix. bex_pool_solve runs one Solve() of every class as one hipGraph launch: one
+ * persistent kernel per class whose waves take systems from a device work queue in the
+ * class's order; bex_pool_solve_periods replays that graph `periods` times without host
+ * synchronisation. Classes of one pool run concurrently. */
+typedef struct bex_pool bex_pool;
+int bex_pool_create(int nclass, const int* units, const int* systems, int device, bex_pool** out);
+int bex_pool_destroy(bex_pool* p);
+int bex_pool_class_solver(bex_pool* p, int c, bex_solver** out);
+/* Coupling matrices of systems first..first+count-1 of class c: count consecutive N x N
+ * column-major matrices (row + col*N), copied synchronously. */
+int bex_pool_set_couplings(bex_pool* p, int c, int first, int count, const double* M);
+/* Order in which class c's systems are handed out (a permutation of 0..NS-1). */
+int bex_pool_set_order(bex_pool* p, int c, const int* order);
+/* Orders class c by the BiCG work its systems have accumulated (counters), largest first. */
+int bex_pool_order_by_cost(bex_pool* p, int c);
+int bex_pool_solve(bex_pool* p);
+int bex_pool_solve_periods(bex_pool* p, int periods);
+int bex_pool_synchronise(bex_pool* p);
+/* Launch variants (lane layout x waves per workgroup) available for a unit count. */
+int bex_pool_variant_count(int units);
+const char* bex_pool_variant_name(int units, int variant);
+int bex_pool_set_variant(bex_pool* p, int c, int variant);
+int bex_pool_get_variant(bex_pool* p, int c, int* variant);
+/* Autotuner: times every variant of every class on the first probe_fraction of the current
+ * time domain (state, counters and time domain restored afterwards) and selects the fastest.
+ * times_ms (optional) receives the times, class by class, variant by variant. */
+int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms);
+const char* bex_pool_last_error(const bex_pool* p);
+
 /* Message of the last failing call on this handle (or of the last bex_create failure when
  * s is NULL). */
 const char* bex_last_error(const bex_solver* s);

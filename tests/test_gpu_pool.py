@@ -1,0 +1,143 @@
+"""GPU tests of pools of uneven ensembles (per-system coupling matrices, persistent kernels with
+device work queues, one hipGraph per period, autotuned launch variants).
+
+Every system of every class is compared bit for bit with the oracle's canonical mode run on
+that system alone with its own coupling matrix: scheduling (queue order, variant, graph replay)
+must not change any result.
+"""
+import numpy as np
+import pytest
+
+from bubbleensemble_implicitextension_amd import collapse
+from bubbleensemble_implicitextension_amd import workloads as wl
+from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+from gpu_helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+UNITS = (8, 16, 32, 64, 128)
+SYSTEMS = (5, 4, 3, 3, 2)
+END = 0.1  # fraction of a period (keeps the N = 128 oracle runs short)
+
+
+def _classes(seed=11):
+    return [wl.config2_class(n, ns, seed=seed) for n, ns in zip(UNITS, SYSTEMS)]
+
+
+def _pool(classes, end=END):
+    pool = collapse.make_pool(classes)
+    for c, e in enumerate(classes):
+        s = pool.solver(c)
+        s.host_view(V.TimeDomain)[e.NS:] = end
+        s.SynchroniseFromHostToDevice(V.All)
+    return pool
+
+
+def _results(pool, classes):
+    pool.synchronise()
+    out = []
+    for c in range(len(classes)):
+        s = pool.solver(c)
+        s.SynchroniseFromDeviceToHost(V.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        out.append({"state": s.host_view(V.ActualState).copy(),
+                    "time": s.host_view(V.ActualTime).copy(),
+                    "acc": s.host_view(V.UnitAccessories).copy(),
+                    "iacc": s.host_view(V.IntegerUnitAccessories).copy(),
+                    "stats": s.stats().copy()})
+    return out
+
+
+def _oracle_system(oracle, ens, sid, periods, end):
+    sub = ens.subset([sid])
+    sub.coupling = sub.coupling[0]
+    p = oracle.OracleProblem.from_ensemble(sub)
+    p.TimeDomain[1:] = end
+    for _ in range(periods):
+        p.solve(mode=oracle.CANONICAL)
+    return p
+
+
+def _check_against_oracle(oracle, classes, res, periods, end):
+    for c, e in enumerate(classes):
+        r = res[c]
+        NS, N = e.NS, e.N
+        state = r["state"].reshape(2, NS, N)
+        acc = r["acc"].reshape(-1, NS, N)
+        iacc = r["iacc"].reshape(-1, NS, N)
+        for sid in range(NS):
+            p = _oracle_system(oracle, e, sid, periods, end)
+            tag = f"N={N} sid={sid}"
+            assert_bitwise(tag + " state", state[:, sid, :], p.ActualState.reshape(2, 1, N)[:, 0])
+            assert_bitwise(tag + " time", r["time"][sid], p.ActualTime[0])
+            assert_bitwise(tag + " acc", acc[:, sid, :], p.UnitAccessories.reshape(-1, 1, N)[:, 0])
+            assert_bitwise(tag + " iacc", iacc[:, sid, :],
+                           p.IntegerUnitAccessories.reshape(-1, 1, N)[:, 0])
+            assert_bitwise(tag + " stats", r["stats"][sid], p.stats_table()[0])
+
+
+def test_pool_matches_oracle_every_variant(oracle):
+    classes = _classes()
+    for v in range(3):
+        print(f"case: pool, variant index {v} (clamped per class), end {END}", flush=True)
+        pool = _pool(classes)
+        for c in range(len(classes)):
+            pool.set_variant(c, min(v, len(pool.variants(c)) - 1))
+        pool.Solve()
+        res = _results(pool, classes)
+        pool.close()
+        _check_against_oracle(oracle, classes, res, 1, END)
+
+
+def test_pool_graph_replays_and_order(oracle):
+    """Three periods as one graph-replay call equal three single-period calls, with reversed
+    and cost-based hand-out orders."""
+    classes = _classes(seed=12)
+    print("case: pool, 3 periods by replay", flush=True)
+    pool = _pool(classes, end=0.05)
+    for c, e in enumerate(classes):
+        pool.set_order(c, np.arange(e.NS)[::-1])
+    pool.solve_periods(3)
+    a = _results(pool, classes)
+    pool.close()
+    print("case: pool, 3 single periods, cost order", flush=True)
+    pool = _pool(classes, end=0.05)
+    for _ in range(3):
+        pool.Solve()
+        pool.synchronise()
+        for c in range(len(classes)):
+            pool.order_by_cost(c)
+    b = _results(pool, classes)
+    pool.close()
+    for c in range(len(classes)):
+        for k in a[c]:
+            assert_bitwise(f"class {c} {k}", a[c][k], b[c][k])
+    _check_against_oracle(oracle, classes, a, 3, 0.05)
+
+
+def test_pool_autotune_restores_state():
+    classes = _classes(seed=13)
+    print("case: pool autotune", flush=True)
+    pool = _pool(classes)
+    pool.synchronise()
+    before = []
+    for c in range(len(classes)):
+        s = pool.solver(c)
+        s.SynchroniseFromDeviceToHost(V.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        before.append(s.host_view(V.ActualState).copy())
+    times = pool.autotune(0.05)
+    for c in range(len(classes)):
+        assert set(times[c]) == set(pool.variants(c))
+        assert all(t > 0 for t in times[c].values())
+        best = min(times[c], key=times[c].get)
+        assert pool.variants(c)[pool.variant(c)] == best
+        s = pool.solver(c)
+        s.SynchroniseFromDeviceToHost(V.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        assert_bitwise(f"class {c} state after autotune", s.host_view(V.ActualState), before[c])
+        assert not s.stats().any()
+    pool.close()
