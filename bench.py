@@ -141,6 +141,64 @@ def error_vs(ref, other):
             "units_event_counts_identical_frac": float(np.mean(same))}
 
 
+def run_pool_leg(end_time, probe_fraction=0.02):
+    """Config 2 (20480 systems, N in {8, 16, 32, 64, 128}, per-system coupling) through a
+    ProblemPool: autotuned variants, one graph launch for the first `end_time` of a period,
+    then the same classes launched one after another for comparison."""
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    classes = wl.config2()
+    pool = collapse.make_pool(classes)
+
+    def reset(pl, cls):
+        for c, e in enumerate(cls):
+            sv = pl.solver(c)
+            collapse.fill(sv, e)
+            sv.host_view(V.TimeDomain)[e.NS:] = end_time
+            sv.SynchroniseFromHostToDevice(V.All)
+            sv.reset_stats()
+        pl.synchronise()
+
+    reset(pool, classes)
+    tuned = pool.autotune(probe_fraction)
+    chosen = {classes[c].N: pool.variants(c)[pool.variant(c)] for c in range(len(classes))}
+    reset(pool, classes)
+    t0 = time.perf_counter()
+    pool.Solve()
+    pool.synchronise()
+    wall = time.perf_counter() - t0
+    stats = [pool.solver(c).stats() for c in range(len(classes))]
+    flops = sum(flops_fm1(st, e.N) for st, e in zip(stats, classes))
+    seq = {}
+    for c, e in enumerate(classes):
+        one = collapse.make_pool([e])
+        one.set_variant(0, pool.variant(c))
+        reset(one, [e])
+        t1 = time.perf_counter()
+        one.Solve()
+        one.synchronise()
+        seq[e.N] = time.perf_counter() - t1
+        one.close()
+    pool.close()
+    ns = sum(e.NS for e in classes)
+    return {
+        "workload": f"config 2: {ns} systems, N in {{8,16,32,64,128}} x 4096, random 3D "
+                    f"positions (50 um min separation), per-system coupling, first {end_time} "
+                    "of a period from rest",
+        "value": ns * end_time / wall, "unit": "system-periods/s (period fraction counted)",
+        "ms": wall * 1e3,
+        "classes_one_after_another_ms": 1e3 * sum(seq.values()),
+        "class_alone_ms": {str(k): v * 1e3 for k, v in seq.items()},
+        "variants": {str(k): v for k, v in chosen.items()},
+        "autotune_probe_ms": {str(classes[c].N): tuned[c] for c in tuned},
+        "bicg_iterations_per_rhs": {str(e.N): float(st[:, 3].sum()) / max(1.0, float(st[:, 2].sum()))
+                                    for st, e in zip(stats, classes)},
+        "fm1_tflops": flops / wall / 1e12,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +207,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm-start leg")
+    ap.add_argument("--pool-end-time", type=float, default=0.1,
+                    help="period fraction of the config-2 pool leg (0 skips it)")
     ap.add_argument("--end-time", type=float, default=1.0,
                     help="fraction of a period per step (profiling only)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -220,6 +280,9 @@ def main():
             "network": "5-64-64-1 BF16 MFMA, trained on a held-out-seed ensemble",
             "error_vs_plain": error_vs(plain, learned),
         }
+    if args.pool_end_time > 0:
+        line["pool"] = run_pool_leg(args.pool_end_time)
+        print(f"# pool leg done ({line['pool']['ms']:.0f} ms)", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
