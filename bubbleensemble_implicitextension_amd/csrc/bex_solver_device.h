@@ -93,360 +93,7 @@ struct BicgOut {
 };
 
 // =========================================================================================
-// Lane-per-unit layouts: GroupLayout<N, W, PS> (N <= 64, 64/N systems per wave, W waves per
-// workgroup) and WideLayout<N> (N = 128, two waves per system). PS = each system has its own
-// coupling matrix (pools of uneven ensembles); otherwise all systems share coupling 0.
-//
-// The coupling matrix lives in LDS. Shared matrices keep M and M^T column-major with stride N;
-// a per-system matrix keeps one copy, column-major with column stride N + 1, which serves both
-// products without bank conflicts: (M v)_i reads M[i + j (N+1)], (M^T v)_i reads M[j + i (N+1)].
-// Either way every row product is the canonical four-partial-sum order.
-
-// A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
-// (Steppers.cuh:489-561), for the lane-per-unit layouts. Returns x in F[1].
-template <class L>
-__device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, double x1, double cpt,
-                                                  double rD, const BexKernelArgs& a) {
-  const double x1sq = x1 * x1;
-  const double xu = F[1];
-  double2* vec = lay.vec;
-  const int u = lay.uid();
-  lay.sync();
-  vec[u].x = cpt;
-  lay.sync();
-  const double b = fma(rD, lay.mv(), xu);
-  lay.sync();
-  vec[u].x = x1sq * xu;
-  lay.sync();
-  const double ax = fma(rD, lay.mv(), xu);
-  double r = -ax + b;
-  double rt = r;
-  double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
-  double rho1 = lay.sum(r * rt);
-  BicgOut out{0, false, true, 0.0};
-  for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
-    if (iter == 1) {
-      p = r;
-      pt = rt;
-      if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
-        out.rho_exit = true;
-        out.maxit = false;
-        break;
-      }
-    } else {
-      const double beta = rho1 / rho2;
-      p = r + beta * p;
-      pt = rt + beta * pt;
-    }
-    lay.sync();
-    vec[u] = make_double2(x1sq * p, rD * pt);
-    lay.sync();
-    const double a1 = lay.mv();
-    const double a2 = lay.mtv();
-    const double q = fma(rD, a1, p);
-    const double qt = fma(x1sq, a2, pt);
-    out.iterations++;
-    const double alpha = rho1 / lay.sum(pt * q);
-    x = x + alpha * p;
-    r = r - alpha * q;
-    rt = rt - alpha * qt;
-    rho2 = rho1;
-    const double rmax = lay.fmax_all(bex::fabs_d(r));
-    rho1 = lay.sum(r * rt);
-    if (rmax < a.BiCGResidualTolerance) {
-      out.maxit = false;
-      break;
-    }
-  }
-  F[1] = x;
-  return out;
-}
-
-// Row products in the canonical order: four partial sums over columns j = q (mod 4), each a
-// sequential fma chain, combined as (s0 + s1) + (s2 + s3). m(j) is the row's j-th element,
-// z(j) the vector's.
-template <int N, class MF, class ZF>
-__device__ __forceinline__ double row_dot4(MF m, ZF z) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll 4
-  for (int j = 0; j < N; j += 4) {
-    s0 = fma(m(j), z(j), s0);
-    s1 = fma(m(j + 1), z(j + 1), s1);
-    s2 = fma(m(j + 2), z(j + 2), s2);
-    s3 = fma(m(j + 3), z(j + 3), s3);
-  }
-  return (s0 + s1) + (s2 + s3);
-}
-
-// Copies system sid's coupling matrix (column-major, stride N, in global memory) into a padded
-// LDS copy (column stride N + 1); lanes u = 0..N-1 of the system's group cooperate.
-template <int N>
-__device__ __forceinline__ void load_padded(double* dst, const double* src, int u, int nthreads) {
-  for (int k = u; k < N * N; k += nthreads) {
-    const int row = k % N, col = k / N;
-    dst[row + col * (N + 1)] = src[k];
-  }
-}
-
-template <int N, int W = 4, bool PS = false>
-struct GroupLayout {
-  static_assert(N == 8 || N == 16 || N == 32 || N == 64, "units per system: 8, 16, 32 or 64");
-  static_assert(W >= 1 && W <= 4, "1 to 4 waves per workgroup");
-  static constexpr int kUnits = N;
-  static constexpr int kSystemsPerWave = kWave / N;
-  static constexpr int kWaves = W;
-  static constexpr int kBlock = kWave * kWaves;
-  static constexpr int kSystemsPerBlock = kSystemsPerWave * kWaves;
-  static constexpr int kWavesPerEU = 4;  // <= 128 VGPRs
-  static constexpr int kMode = kPlain;
-  static constexpr bool kPerSystemM = PS;
-  static constexpr int kItemsPerBlock = W;  // persistent kernels: each wave pops its own item
-  static constexpr int kSystemsPerItem = kSystemsPerWave;
-  static constexpr int kLd = PS ? N + 1 : N;
-
-  static constexpr size_t m_bytes() {
-    return PS ? sizeof(double) * kSystemsPerBlock * N * (N + 1) : sizeof(double) * 2 * N * N;
-  }
-  static constexpr size_t lds_bytes() {
-    return m_bytes() + sizeof(double2) * kSystemsPerBlock * N + sizeof(Sys) * kSystemsPerBlock +
-           sizeof(int) * kSystemsPerBlock * BEX_NSTAT;
-  }
-
-  int lane, grp, uid_, lsys, wave, sid_;
-  const double* sM;   // shared: M; per-system: this system's padded copy
-  const double* sMT;  // shared: M^T
-  double* myM;
-  double2* vec;
-  Sys* sys;
-  int* stat;
-
-  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
-    const int tid = threadIdx.x;
-    lane = tid & (kWave - 1);
-    wave = tid / kWave;
-    grp = lane / N;
-    uid_ = lane % N;
-    lsys = wave * kSystemsPerWave + grp;
-    double* m = reinterpret_cast<double*>(smem);
-    double2* v = reinterpret_cast<double2*>(smem + m_bytes());
-    Sys* s = reinterpret_cast<Sys*>(v + kSystemsPerBlock * N);
-    int* st = reinterpret_cast<int*>(s + kSystemsPerBlock);
-    if (PS) {
-      myM = m + (size_t)lsys * N * (N + 1);
-      sM = myM;
-      sMT = myM;
-    } else {
-      double* mt = m + N * N;
-      for (int k = tid; k < N * N; k += kBlock) {
-        m[k] = a.CouplingMatrix[k];
-        mt[k] = a.CouplingMatrixT[k];
-      }
-      __syncthreads();  // the only workgroup barrier
-      sM = m;
-      sMT = mt;
-      myM = nullptr;
-    }
-    vec = v + lsys * N;
-    sys = s + lsys;
-    stat = st + lsys * BEX_NSTAT;
-  }
-
-  // Assigns this lane's system for work item `item` (one item = the systems of one wave):
-  // order == nullptr -> systems first_system + item * kSystemsPerWave + grp; otherwise the
-  // order array's entries. Loads a per-system matrix and clears the group's counters.
-  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
-                                       long long count) {
-    const long long pos = item * kSystemsPerWave + grp;
-    int s = -1;
-    if (order) {
-      if (pos < count) s = order[pos];
-    } else if (pos < count) {
-      s = a.first_system + (int)pos;
-    }
-    sid_ = s;
-    for (int k = uid_; k < BEX_NSTAT; k += N) stat[k] = 0;
-    if (PS && s >= 0 && s < a.NS)
-      load_padded<N>(myM, a.CouplingMatrix + (size_t)s * N * N, uid_, N);
-    wave_sync();
-  }
-
-  __device__ __forceinline__ int sid() const { return sid_; }
-  __device__ __forceinline__ int uid() const { return uid_; }
-  __device__ __forceinline__ bool primal() const { return true; }
-  __device__ __forceinline__ void sync() const { wave_sync(); }
-
-  __device__ __forceinline__ unsigned long long mask() const {
-    return (N == 64) ? ~0ull : (((1ull << N) - 1ull) << (grp * N));
-  }
-  __device__ __forceinline__ bool all(bool p) const { return (__ballot(p) & mask()) == mask(); }
-  __device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask()) != 0ull; }
-
-  // Halving-tree sum (Steppers.cuh:733-742) as an xor butterfly: every lane ends with the
-  // tree's association (IEEE addition is commutative).
-  __device__ __forceinline__ double sum(double v) const {
-#pragma unroll
-    for (int h = N / 2; h >= 1; h >>= 1) v = v + __shfl_xor(v, h, kWave);
-    return v;
-  }
-  __device__ __forceinline__ double fmax_all(double v) const {
-#pragma unroll
-    for (int h = N / 2; h >= 1; h >>= 1) v = bex::fmax_d(v, __shfl_xor(v, h, kWave));
-    return v;
-  }
-  // Minimum where a NaN candidate never replaces a value (MathFunction.cuh:54-64 atomicFMIN).
-  __device__ __forceinline__ double min_keep(double v) const {
-#pragma unroll
-    for (int h = N / 2; h >= 1; h >>= 1) {
-      const double o = __shfl_xor(v, h, kWave);
-      v = (o < v) ? o : v;
-    }
-    return v;
-  }
-
-  __device__ __forceinline__ double mv() const {
-    const double* m = sM;
-    const double2* z = vec;
-    const int i = uid_;
-    return row_dot4<N>([&](int j) { return m[i + j * kLd]; }, [&](int j) { return z[j].x; });
-  }
-  __device__ __forceinline__ double mtv() const {
-    const double2* z = vec;
-    const int i = uid_;
-    if (PS) {
-      const double* m = sM;
-      return row_dot4<N>([&](int j) { return m[j + i * kLd]; }, [&](int j) { return z[j].y; });
-    }
-    const double* mt = sMT;
-    return row_dot4<N>([&](int j) { return mt[i + j * N]; }, [&](int j) { return z[j].y; });
-  }
-
-  template <class Feat>
-  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
-                                              const BexKernelArgs& a, Feat&&, bool) const {
-    return group_coupling(*this, F, x1, cpt, rD, a);
-  }
-};
-
-// N = 128: one system per workgroup of two waves, lane = unit. Reductions combine the halves
-// through LDS (first tree level s[i] + s[i + 64]) and finish with an in-wave butterfly that
-// both waves evaluate identically; system-scope synchronisation is the workgroup barrier.
-template <int N, bool PS = true>
-struct WideLayout {
-  static_assert(N == 128, "two-wave layout: 128 units");
-  static constexpr int kUnits = N;
-  static constexpr int kWaves = N / kWave;
-  static constexpr int kBlock = N;
-  static constexpr int kSystemsPerBlock = 1;
-  static constexpr int kWavesPerEU = 4;
-  static constexpr int kMode = kPlain;
-  static constexpr bool kPerSystemM = PS;
-  static constexpr int kItemsPerBlock = 1;
-  static constexpr int kSystemsPerItem = 1;
-  static constexpr int kLd = N + 1;
-
-  static constexpr size_t lds_bytes() {
-    return sizeof(double) * N * (N + 1) + sizeof(double2) * N + sizeof(double) * N +
-           sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
-  }
-
-  int tid, sid_;
-  double* sM;
-  double2* vec;
-  double* red;
-  Sys* sys;
-  int* stat;
-  int* flags;
-
-  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
-    tid = threadIdx.x;
-    sM = reinterpret_cast<double*>(smem);
-    vec = reinterpret_cast<double2*>(sM + N * (N + 1));
-    red = reinterpret_cast<double*>(vec + N);
-    sys = reinterpret_cast<Sys*>(red + N);
-    stat = reinterpret_cast<int*>(sys + 1);
-    flags = stat + BEX_NSTAT;
-    if (!PS) {
-      load_padded<N>(sM, a.CouplingMatrix, tid, kBlock);
-      __syncthreads();
-    }
-  }
-
-  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
-                                       long long count) {
-    int s = -1;
-    if (item < count) s = order ? order[item] : a.first_system + (int)item;
-    sid_ = s;
-    if (tid < BEX_NSTAT) stat[tid] = 0;
-    if (PS && s >= 0 && s < a.NS) load_padded<N>(sM, a.CouplingMatrix + (size_t)s * N * N, tid, kBlock);
-    __syncthreads();
-  }
-
-  __device__ __forceinline__ int sid() const { return sid_; }
-  __device__ __forceinline__ int uid() const { return tid; }
-  __device__ __forceinline__ bool primal() const { return true; }
-  __device__ __forceinline__ void sync() const { __syncthreads(); }
-
-  template <class Op>
-  __device__ __forceinline__ double reduce(double v, Op op) const {
-    red[tid] = v;
-    __syncthreads();
-    const int l = tid & (kWave - 1);
-    double w = op(red[l], red[l + kWave]);
-#pragma unroll
-    for (int h = kWave / 2; h >= 1; h >>= 1) w = op(w, __shfl_xor(w, h, kWave));
-    __syncthreads();
-    return w;
-  }
-  __device__ __forceinline__ double sum(double v) const {
-    return reduce(v, [](double x, double y) { return x + y; });
-  }
-  __device__ __forceinline__ double fmax_all(double v) const {
-    return reduce(v, [](double x, double y) { return bex::fmax_d(x, y); });
-  }
-  __device__ __forceinline__ double min_keep(double v) const {
-    return reduce(v, [](double x, double y) { return (y < x) ? y : x; });
-  }
-  __device__ __forceinline__ bool all(bool p) const {
-    if (tid == 0) flags[0] = 0;
-    __syncthreads();
-    if (!p) flags[0] = 1;
-    __syncthreads();
-    const bool r = flags[0] == 0;
-    __syncthreads();
-    return r;
-  }
-  __device__ __forceinline__ bool any(bool p) const {
-    if (tid == 0) flags[1] = 0;
-    __syncthreads();
-    if (p) flags[1] = 1;
-    __syncthreads();
-    const bool r = flags[1] != 0;
-    __syncthreads();
-    return r;
-  }
-
-  __device__ __forceinline__ double mv() const {
-    const double* m = sM;
-    const double2* z = vec;
-    const int i = tid;
-    return row_dot4<N>([&](int j) { return m[i + j * kLd]; }, [&](int j) { return z[j].x; });
-  }
-  __device__ __forceinline__ double mtv() const {
-    const double* m = sM;
-    const double2* z = vec;
-    const int i = tid;
-    return row_dot4<N>([&](int j) { return m[j + i * kLd]; }, [&](int j) { return z[j].y; });
-  }
-
-  template <class Feat>
-  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
-                                              const BexKernelArgs& a, Feat&&, bool) const {
-    return group_coupling(*this, F, x1, cpt, rD, a);
-  }
-};
-
-// =========================================================================================
-// RolePairLayout32: one N = 32 system per wave, primal and shadow halves.
+// Cross-lane building blocks (DPP, permlane swaps) shared by the layouts.
 
 // acc[j % 4] = fma(z(lane j of the lane's 16-lane row), m[j], acc[j % 4]) for j = 0..15, in
 // order: the canonical four-partial-sum order of every matrix-vector product. The s_nop covers
@@ -538,6 +185,402 @@ struct FmaxOp {
     return bex::fmax_d(a, b);
   }
 };
+// Minimum where a NaN candidate never replaces a value (MathFunction.cuh:54-64 atomicFMIN).
+struct MinKeepOp {
+  __device__ __forceinline__ double operator()(double a, double b) const { return (b < a) ? b : a; }
+};
+
+// =========================================================================================
+// Lane-per-unit layouts: GroupLayout<N, W, PS> (N <= 64, 64/N systems per wave, W waves per
+// workgroup) and WideLayout<N> (N = 128, two waves per system). PS = each system has its own
+// coupling matrix (pools of uneven ensembles); otherwise all systems share coupling 0.
+//
+// The coupling matrix lives in LDS. Shared matrices keep M and M^T column-major with stride N;
+// a per-system matrix keeps one copy, column-major with column stride N + 1, which serves both
+// products without bank conflicts: (M v)_i reads M[i + j (N+1)], (M^T v)_i reads M[j + i (N+1)].
+// Either way every row product is the canonical four-partial-sum order.
+
+// A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
+// (Steppers.cuh:489-561), for the lane-per-unit layouts. Returns x in F[1].
+template <class L>
+__device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, double x1, double cpt,
+                                                  double rD, const BexKernelArgs& a) {
+  const double x1sq = x1 * x1;
+  const double xu = F[1];
+  double2* vec = lay.vec;
+  const int u = lay.uid();
+  lay.sync();
+  vec[u].x = cpt;
+  lay.sync();
+  const double b = fma(rD, lay.mv(), xu);
+  lay.sync();
+  vec[u].x = x1sq * xu;
+  lay.sync();
+  const double ax = fma(rD, lay.mv(), xu);
+  double r = -ax + b;
+  double rt = r;
+  double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
+  double rho1 = lay.sum(r * rt);
+  BicgOut out{0, false, true, 0.0};
+  for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
+    if (iter == 1) {
+      p = r;
+      pt = rt;
+      if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
+        out.rho_exit = true;
+        out.maxit = false;
+        break;
+      }
+    } else {
+      const double beta = rho1 / rho2;
+      p = r + beta * p;
+      pt = rt + beta * pt;
+    }
+    lay.sync();
+    vec[u] = make_double2(x1sq * p, rD * pt);
+    lay.sync();
+    const double a1 = lay.mv();
+    const double a2 = lay.mtv();
+    const double q = fma(rD, a1, p);
+    const double qt = fma(x1sq, a2, pt);
+    out.iterations++;
+    const double alpha = rho1 / lay.sum(pt * q);
+    x = x + alpha * p;
+    r = r - alpha * q;
+    rt = rt - alpha * qt;
+    rho2 = rho1;
+    const double rmax = lay.fmax_all(bex::fabs_d(r));
+    rho1 = lay.sum(r * rt);
+    if (rmax < a.BiCGResidualTolerance) {
+      out.maxit = false;
+      break;
+    }
+  }
+  F[1] = x;
+  return out;
+}
+
+// Row products in the canonical order: four partial sums over columns j = q (mod 4), each a
+// sequential fma chain, combined as (s0 + s1) + (s2 + s3). m(j) is the row's j-th element,
+// z(j) the vector's.
+// Columns are fetched C at a time into registers before their FMAs, so C LDS reads are in
+// flight at once instead of one per FMA.
+template <int N, int C, class MF, class ZF>
+__device__ __forceinline__ double row_dot4(MF m, ZF z) {
+  static_assert(C % 4 == 0 && N % C == 0, "chunk of whole column quadruples");
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int j0 = 0; j0 < N; j0 += C) {
+    double mm[C], zz[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      mm[k] = m(j0 + k);
+      zz[k] = z(j0 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) s[k & 3] = fma(mm[k], zz[k], s[k & 3]);
+  }
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// Copies system sid's coupling matrix (column-major, stride N, in global memory) into a padded
+// LDS copy (column stride N + 1); lanes u = 0..N-1 of the system's group cooperate.
+template <int N>
+__device__ __forceinline__ void load_padded(double* dst, const double* src, int u, int nthreads) {
+  for (int k = u; k < N * N; k += nthreads) {
+    const int row = k % N, col = k / N;
+    dst[row + col * (N + 1)] = src[k];
+  }
+}
+
+// Halving tree (Steppers.cuh:733-742) over the N lanes of each group of a wave, N = 8..64:
+// s[i] = op(s[i], s[i + h]) for h = N/2, ..., 1, evaluated with cross-lane moves that keep
+// that pairing (permlane32 swap for h = 32, permlane16 swap for h = 16, DPP row_shl for
+// h <= 8); the result is the group's first lane's, broadcast to the group.
+template <int N, class Op>
+__device__ __forceinline__ double group_tree(double v, Op op, int grp) {
+  if (N == 64) {
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    v = op(v, __hiloint2double((int)h[1], (int)l[1]));
+  }
+  if (N >= 32) {
+    double even, odd;
+    rows_of_half(v, &even, &odd);
+    v = op(v, odd);
+  }
+  if (N >= 16) v = op(v, __builtin_amdgcn_mov_dpp(v, 0x108, 0xf, 0xf, true));  // row_shl:8
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x104, 0xf, 0xf, true));              // row_shl:4
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x102, 0xf, 0xf, true));              // row_shl:2
+  v = op(v, __builtin_amdgcn_mov_dpp(v, 0x101, 0xf, 0xf, true));              // row_shl:1
+  if (N == 64) return uniform(v);
+  if (N == 32) {  // lanes 0 and 32 hold the two groups' results
+    const int alo = __builtin_amdgcn_readlane((int)__double2loint(v), 0);
+    const int ahi = __builtin_amdgcn_readlane((int)__double2hiint(v), 0);
+    const int blo = __builtin_amdgcn_readlane((int)__double2loint(v), 32);
+    const int bhi = __builtin_amdgcn_readlane((int)__double2hiint(v), 32);
+    return grp == 0 ? __hiloint2double(ahi, alo) : __hiloint2double(bhi, blo);
+  }
+  if (N == 16) {  // row_newbcast:0: lane 0 of each 16-lane row to the row
+    const int lo = __builtin_amdgcn_mov_dpp((int)__double2loint(v), 0x150, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)__double2hiint(v), 0x150, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+  }
+  return __shfl(v, grp * N, kWave);  // N = 8
+}
+
+template <int N, int W = 4, bool PS = false>
+struct GroupLayout {
+  static_assert(N == 8 || N == 16 || N == 32 || N == 64, "units per system: 8, 16, 32 or 64");
+  static_assert(W >= 1 && W <= 4, "1 to 4 waves per workgroup");
+  static constexpr int kUnits = N;
+  static constexpr int kSystemsPerWave = kWave / N;
+  static constexpr int kWaves = W;
+  static constexpr int kBlock = kWave * kWaves;
+  static constexpr int kSystemsPerBlock = kSystemsPerWave * kWaves;
+  // Per-system matrices bound the occupancy through LDS (N = 64: 33 KB per wave, at most four
+  // waves per CU); give those kernels the registers instead.
+  static constexpr int kWavesPerEU = (PS && N == 64) ? 1 : (PS && N == 32) ? 2 : 4;
+  static constexpr int kChunk = (kWavesPerEU <= 2) ? 16 : 8;
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = W;  // persistent kernels: each wave pops its own item
+  static constexpr int kSystemsPerItem = kSystemsPerWave;
+  static constexpr int kLd = PS ? N + 1 : N;
+
+  static constexpr size_t m_bytes() {
+    return PS ? sizeof(double) * kSystemsPerBlock * N * (N + 1) : sizeof(double) * 2 * N * N;
+  }
+  static constexpr size_t lds_bytes() {
+    return m_bytes() + sizeof(double2) * kSystemsPerBlock * N + sizeof(Sys) * kSystemsPerBlock +
+           sizeof(int) * kSystemsPerBlock * BEX_NSTAT;
+  }
+
+  int lane, grp, uid_, lsys, wave, sid_;
+  const double* sM;   // shared: M; per-system: this system's padded copy
+  const double* sMT;  // shared: M^T
+  double* myM;
+  double2* vec;
+  Sys* sys;
+  int* stat;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
+    const int tid = threadIdx.x;
+    lane = tid & (kWave - 1);
+    wave = tid / kWave;
+    grp = lane / N;
+    uid_ = lane % N;
+    lsys = wave * kSystemsPerWave + grp;
+    double* m = reinterpret_cast<double*>(smem);
+    double2* v = reinterpret_cast<double2*>(smem + m_bytes());
+    Sys* s = reinterpret_cast<Sys*>(v + kSystemsPerBlock * N);
+    int* st = reinterpret_cast<int*>(s + kSystemsPerBlock);
+    if (PS) {
+      myM = m + (size_t)lsys * N * (N + 1);
+      sM = myM;
+      sMT = myM;
+    } else {
+      double* mt = m + N * N;
+      for (int k = tid; k < N * N; k += kBlock) {
+        m[k] = a.CouplingMatrix[k];
+        mt[k] = a.CouplingMatrixT[k];
+      }
+      __syncthreads();  // the only workgroup barrier
+      sM = m;
+      sMT = mt;
+      myM = nullptr;
+    }
+    vec = v + lsys * N;
+    sys = s + lsys;
+    stat = st + lsys * BEX_NSTAT;
+  }
+
+  // Assigns this lane's system for work item `item` (one item = the systems of one wave):
+  // order == nullptr -> systems first_system + item * kSystemsPerWave + grp; otherwise the
+  // order array's entries. Loads a per-system matrix and clears the group's counters.
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    const long long pos = item * kSystemsPerWave + grp;
+    int s = -1;
+    if (order) {
+      if (pos < count) s = order[pos];
+    } else if (pos < count) {
+      s = a.first_system + (int)pos;
+    }
+    sid_ = s;
+    for (int k = uid_; k < BEX_NSTAT; k += N) stat[k] = 0;
+    if (PS && s >= 0 && s < a.NS)
+      load_padded<N>(myM, a.CouplingMatrix + (size_t)s * N * N, uid_, N);
+    wave_sync();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return uid_; }
+  __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+
+  __device__ __forceinline__ unsigned long long mask() const {
+    return (N == 64) ? ~0ull : (((1ull << N) - 1ull) << (grp * N));
+  }
+  __device__ __forceinline__ bool all(bool p) const { return (__ballot(p) & mask()) == mask(); }
+  __device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask()) != 0ull; }
+
+  // Halving-tree reductions (Steppers.cuh:733-742) of the group, broadcast to the group.
+  __device__ __forceinline__ double sum(double v) const { return group_tree<N>(v, AddOp(), grp); }
+  __device__ __forceinline__ double fmax_all(double v) const {
+    return group_tree<N>(v, FmaxOp(), grp);
+  }
+  // Minimum where a NaN candidate never replaces a value (MathFunction.cuh:54-64 atomicFMIN).
+  __device__ __forceinline__ double min_keep(double v) const {
+    return group_tree<N>(v, MinKeepOp(), grp);
+  }
+
+  __device__ __forceinline__ double mv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = uid_;
+    return row_dot4<N, kChunk>([&](int j) { return m[i + j * kLd]; },
+                               [&](int j) { return z[j].x; });
+  }
+  __device__ __forceinline__ double mtv() const {
+    const double2* z = vec;
+    const int i = uid_;
+    if (PS) {
+      const double* m = sM;
+      return row_dot4<N, kChunk>([&](int j) { return m[j + i * kLd]; },
+                                 [&](int j) { return z[j].y; });
+    }
+    const double* mt = sMT;
+    return row_dot4<N, kChunk>([&](int j) { return mt[i + j * N]; },
+                               [&](int j) { return z[j].y; });
+  }
+
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    return group_coupling(*this, F, x1, cpt, rD, a);
+  }
+};
+
+// N = 128: one system per workgroup of two waves, lane = unit. Reductions combine the halves
+// through LDS (first tree level s[i] + s[i + 64]) and finish with an in-wave butterfly that
+// both waves evaluate identically; system-scope synchronisation is the workgroup barrier.
+template <int N, bool PS = true>
+struct WideLayout {
+  static_assert(N == 128, "two-wave layout: 128 units");
+  static constexpr int kUnits = N;
+  static constexpr int kWaves = N / kWave;
+  static constexpr int kBlock = N;
+  static constexpr int kSystemsPerBlock = 1;
+  static constexpr int kWavesPerEU = 1;  // LDS holds one system per CU: use the registers
+  static constexpr int kChunk = 16;
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 1;
+  static constexpr int kLd = N + 1;
+
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * N * (N + 1) + sizeof(double2) * N + sizeof(double) * N +
+           sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
+  }
+
+  int tid, sid_;
+  double* sM;
+  double2* vec;
+  double* red;
+  Sys* sys;
+  int* stat;
+  int* flags;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
+    tid = threadIdx.x;
+    sM = reinterpret_cast<double*>(smem);
+    vec = reinterpret_cast<double2*>(sM + N * (N + 1));
+    red = reinterpret_cast<double*>(vec + N);
+    sys = reinterpret_cast<Sys*>(red + N);
+    stat = reinterpret_cast<int*>(sys + 1);
+    flags = stat + BEX_NSTAT;
+    if (!PS) {
+      load_padded<N>(sM, a.CouplingMatrix, tid, kBlock);
+      __syncthreads();
+    }
+  }
+
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    int s = -1;
+    if (item < count) s = order ? order[item] : a.first_system + (int)item;
+    sid_ = s;
+    if (tid < BEX_NSTAT) stat[tid] = 0;
+    if (PS && s >= 0 && s < a.NS) load_padded<N>(sM, a.CouplingMatrix + (size_t)s * N * N, tid, kBlock);
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return tid; }
+  __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+
+  // First tree level s[i] = op(s[i], s[i + 64]) through LDS, the rest in-wave; both waves
+  // evaluate the same 64-lane tree and end with the same uniform value.
+  template <class Op>
+  __device__ __forceinline__ double reduce(double v, Op op) const {
+    red[tid] = v;
+    __syncthreads();
+    const int l = tid & (kWave - 1);
+    const double w = group_tree<kWave>(op(red[l], red[l + kWave]), op, 0);
+    __syncthreads();
+    return w;
+  }
+  __device__ __forceinline__ double sum(double v) const { return reduce(v, AddOp()); }
+  __device__ __forceinline__ double fmax_all(double v) const { return reduce(v, FmaxOp()); }
+  __device__ __forceinline__ double min_keep(double v) const { return reduce(v, MinKeepOp()); }
+  __device__ __forceinline__ bool all(bool p) const {
+    if (tid == 0) flags[0] = 0;
+    __syncthreads();
+    if (!p) flags[0] = 1;
+    __syncthreads();
+    const bool r = flags[0] == 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool any(bool p) const {
+    if (tid == 0) flags[1] = 0;
+    __syncthreads();
+    if (p) flags[1] = 1;
+    __syncthreads();
+    const bool r = flags[1] != 0;
+    __syncthreads();
+    return r;
+  }
+
+  __device__ __forceinline__ double mv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = tid;
+    return row_dot4<N, kChunk>([&](int j) { return m[i + j * kLd]; },
+                               [&](int j) { return z[j].x; });
+  }
+  __device__ __forceinline__ double mtv() const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = tid;
+    return row_dot4<N, kChunk>([&](int j) { return m[j + i * kLd]; },
+                               [&](int j) { return z[j].y; });
+  }
+
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    return group_coupling(*this, F, x1, cpt, rD, a);
+  }
+};
+
+// =========================================================================================
+// RolePairLayout32: one N = 32 system per wave, primal and shadow halves.
+
 
 typedef short bex_v4s __attribute__((ext_vector_type(4)));
 typedef float bex_v16f __attribute__((ext_vector_type(16)));
