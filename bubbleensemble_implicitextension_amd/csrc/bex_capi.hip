@@ -50,6 +50,7 @@ struct bex_solver {
   double* d_CouplingFullT = nullptr;     // transpose of coupling 0
   bool PerSystemCoupling = false;        // pool classes: one full matrix per system
   double* d_CouplingSys = nullptr;       // [NS][N][N] column-major (pool classes)
+  double* d_CouplingSysT = nullptr;      // their transposes (pool classes that stream M)
   double* h_tol = nullptr;               // [2*UD + NE] rtol, atol, event tolerance
   int* h_dir = nullptr;                  // [NE]
   double* d_tol = nullptr;
@@ -140,6 +141,7 @@ bool kernel_for(int ups, int alg, int layout, int mode, KernelChoice* k) {
     case 32: *k = choice<bexk::GroupLayout<32>>(alg); return true;
     case 64: *k = choice<bexk::GroupLayout<64>>(alg); return true;
     case 128: *k = choice<bexk::WideLayout<128, false>>(alg); return true;
+    case 256: *k = choice<bexk::WideLayout<256, false>>(alg); return true;
     default: return false;
   }
 }
@@ -184,7 +186,7 @@ void fill_args(const bex_solver* s, BexKernelArgs* ap) {
   a.SystemAccessories = (double*)s->SystemAccessories.d;
   a.IntegerSystemAccessories = (int*)s->IntegerSystemAccessories.d;
   a.CouplingMatrix = s->PerSystemCoupling ? s->d_CouplingSys : s->d_CouplingFull;
-  a.CouplingMatrixT = s->PerSystemCoupling ? nullptr : s->d_CouplingFullT;
+  a.CouplingMatrixT = s->PerSystemCoupling ? s->d_CouplingSysT : s->d_CouplingFullT;
   a.DenseOutputIndex = (int*)s->DenseOutputIndex.d;
   a.DenseOutputTimeInstances = (double*)s->DenseOutputTimeInstances.d;
   a.DenseOutputStates = (double*)s->DenseOutputStates.d;
@@ -366,7 +368,7 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   KernelChoice kc;
   if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
-                "bex_create: units per system must be 8, 16, 32, 64 or 128 in this build");
+                "bex_create: units per system must be 8, 16, 32, 64, 128 or 256 in this build");
 
   bex_solver* s = new bex_solver();
   s->d = d;
@@ -496,6 +498,7 @@ int bex_destroy(bex_solver* s) {
   if (s->d_CouplingFull) hipFree(s->d_CouplingFull);
   if (s->d_CouplingFullT) hipFree(s->d_CouplingFullT);
   if (s->d_CouplingSys) hipFree(s->d_CouplingSys);
+  if (s->d_CouplingSysT) hipFree(s->d_CouplingSysT);
   if (s->h_tol) hipHostFree(s->h_tol);
   if (s->h_dir) hipHostFree(s->h_dir);
   if (s->d_tol) hipFree(s->d_tol);
@@ -1067,6 +1070,8 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<64, 2, true>>("group64 x2 waves")};
   static const std::vector<PoolVariant> v128 = {
       pool_variant<bexk::WideLayout<128, true>>("wide128 x2 waves")};
+  static const std::vector<PoolVariant> v256 = {
+      pool_variant<bexk::WideLayout<256, true>>("wide256 x4 waves, M streamed")};
   static const std::vector<PoolVariant> none;
   switch (n) {
     case 8: return v8;
@@ -1074,6 +1079,7 @@ const std::vector<PoolVariant>& pool_variants(int n) {
     case 32: return v32;
     case 64: return v64;
     case 128: return v128;
+    case 256: return v256;
     default: return none;
   }
 }
@@ -1221,7 +1227,8 @@ int bex_pool_create(int nclass, const int* units, const int* systems, int device
   for (int c = 0; c < nclass; c++) {
     const int N = units[c], NS = systems[c];
     if (pool_variants(N).empty() || NS <= 0)
-      return bail(BEX_E_UNSUPPORTED, "bex_pool_create: units per system must be 8, 16, 32, 64 or 128");
+      return bail(BEX_E_UNSUPPORTED,
+                  "bex_pool_create: units per system must be 8, 16, 32, 64, 128 or 256");
     bex_problem_desc d = dims;
     d.NS = NS;
     d.UPS = N;
@@ -1247,6 +1254,9 @@ int bex_pool_create(int nclass, const int* units, const int* systems, int device
         hipMalloc((void**)&kk.d_order, sizeof(int) * (size_t)NS) != hipSuccess ||
         hipMemcpy(kk.d_order, order.data(), sizeof(int) * (size_t)NS, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc((void**)&kk.d_counter, sizeof(unsigned long long)) != hipSuccess)
+      return bail(BEX_E_MEMORY, "bex_pool_create: not enough device memory");
+    if (N > 128 && (hipMalloc((void**)&kk.s->d_CouplingSysT, sizeof(double) * (size_t)NS * N * N) != hipSuccess ||
+                    hipMemset(kk.s->d_CouplingSysT, 0, sizeof(double) * (size_t)NS * N * N) != hipSuccess))
       return bail(BEX_E_MEMORY, "bex_pool_create: not enough device memory");
     hipStream_t st = nullptr;
     hipEvent_t j = nullptr, r = nullptr;
@@ -1304,6 +1314,16 @@ int bex_pool_set_couplings(bex_pool* p, int c, int first, int count, const doubl
   const size_t nn = (size_t)k.N * k.N;
   BEX_PHIP(p, hipMemcpy(k.s->d_CouplingSys + (size_t)first * nn, M, sizeof(double) * nn * count,
                         hipMemcpyHostToDevice));
+  if (k.s->d_CouplingSysT) {  // transposes for the streamed layouts
+    std::vector<double> t(nn);
+    for (int q = 0; q < count; q++) {
+      const double* src = M + (size_t)q * nn;
+      for (int r = 0; r < k.N; r++)
+        for (int c2 = 0; c2 < k.N; c2++) t[(size_t)c2 + (size_t)r * k.N] = src[(size_t)r + (size_t)c2 * k.N];
+      BEX_PHIP(p, hipMemcpy(k.s->d_CouplingSysT + (size_t)(first + q) * nn, t.data(),
+                            sizeof(double) * nn, hipMemcpyHostToDevice));
+    }
+  }
   return BEX_OK;
 }
 

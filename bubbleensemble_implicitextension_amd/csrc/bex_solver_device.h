@@ -463,17 +463,22 @@ struct GroupLayout {
   }
 };
 
-// N = 128: one system per workgroup of two waves, lane = unit. Reductions combine the halves
-// through LDS (first tree level s[i] + s[i + 64]) and finish with an in-wave butterfly that
-// both waves evaluate identically; system-scope synchronisation is the workgroup barrier.
+// N = 128 and 256: one system per workgroup of N / 64 waves, lane = unit. Reductions combine
+// the waves through LDS (the tree's cross-wave levels, s[i] op s[i + 64] for N = 128 and
+// (s[i] op s[i + 128]) op (s[i + 64] op s[i + 192]) for N = 256) and finish with an in-wave
+// tree that every wave evaluates identically; system-scope synchronisation is the workgroup
+// barrier. N = 128 keeps the system's matrix in LDS (132 KB, one system per CU); a 256 x 256
+// FP64 matrix (512 KB) exceeds the CU's LDS, so N = 256 streams M and M^T (column-major, both
+// products coalesced) from global memory through L2 and MALL.
 template <int N, bool PS = true>
 struct WideLayout {
-  static_assert(N == 128, "two-wave layout: 128 units");
+  static_assert(N == 128 || N == 256, "multi-wave layout: 128 or 256 units");
+  static constexpr bool kLdsM = (N == 128);
   static constexpr int kUnits = N;
   static constexpr int kWaves = N / kWave;
   static constexpr int kBlock = N;
   static constexpr int kSystemsPerBlock = 1;
-  static constexpr int kWavesPerEU = 1;  // LDS holds one system per CU: use the registers
+  static constexpr int kWavesPerEU = kLdsM ? 1 : 2;  // N = 128: LDS holds one system per CU
   static constexpr int kChunk = 16;
   static constexpr int kMode = kPlain;
   static constexpr bool kPerSystemM = PS;
@@ -482,12 +487,14 @@ struct WideLayout {
   static constexpr int kLd = N + 1;
 
   static constexpr size_t lds_bytes() {
-    return sizeof(double) * N * (N + 1) + sizeof(double2) * N + sizeof(double) * N +
+    return (kLdsM ? sizeof(double) * N * (N + 1) : 0) + sizeof(double2) * N + sizeof(double) * N +
            sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
   }
 
   int tid, sid_;
-  double* sM;
+  double* sM;         // N = 128: padded LDS copy
+  const double* gM;   // N = 256: the system's M, column-major, global memory
+  const double* gMT;  // N = 256: its transpose
   double2* vec;
   double* red;
   Sys* sys;
@@ -496,13 +503,17 @@ struct WideLayout {
 
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     tid = threadIdx.x;
-    sM = reinterpret_cast<double*>(smem);
-    vec = reinterpret_cast<double2*>(sM + N * (N + 1));
+    unsigned char* p = smem;
+    sM = reinterpret_cast<double*>(p);
+    if (kLdsM) p += sizeof(double) * N * (N + 1);
+    vec = reinterpret_cast<double2*>(p);
     red = reinterpret_cast<double*>(vec + N);
     sys = reinterpret_cast<Sys*>(red + N);
     stat = reinterpret_cast<int*>(sys + 1);
     flags = stat + BEX_NSTAT;
-    if (!PS) {
+    gM = a.CouplingMatrix;
+    gMT = a.CouplingMatrixT;
+    if (!PS && kLdsM) {
       load_padded<N>(sM, a.CouplingMatrix, tid, kBlock);
       __syncthreads();
     }
@@ -514,7 +525,14 @@ struct WideLayout {
     if (item < count) s = order ? order[item] : a.first_system + (int)item;
     sid_ = s;
     if (tid < BEX_NSTAT) stat[tid] = 0;
-    if (PS && s >= 0 && s < a.NS) load_padded<N>(sM, a.CouplingMatrix + (size_t)s * N * N, tid, kBlock);
+    if (PS && s >= 0 && s < a.NS) {
+      if (kLdsM) {
+        load_padded<N>(sM, a.CouplingMatrix + (size_t)s * N * N, tid, kBlock);
+      } else {
+        gM = a.CouplingMatrix + (size_t)s * N * N;
+        gMT = a.CouplingMatrixT + (size_t)s * N * N;
+      }
+    }
     __syncthreads();
   }
 
@@ -523,14 +541,14 @@ struct WideLayout {
   __device__ __forceinline__ bool primal() const { return true; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 
-  // First tree level s[i] = op(s[i], s[i + 64]) through LDS, the rest in-wave; both waves
-  // evaluate the same 64-lane tree and end with the same uniform value.
   template <class Op>
   __device__ __forceinline__ double reduce(double v, Op op) const {
     red[tid] = v;
     __syncthreads();
     const int l = tid & (kWave - 1);
-    const double w = group_tree<kWave>(op(red[l], red[l + kWave]), op, 0);
+    const double first = (N == 128) ? op(red[l], red[l + 64])
+                                    : op(op(red[l], red[l + 128]), op(red[l + 64], red[l + 192]));
+    const double w = group_tree<kWave>(first, op, 0);
     __syncthreads();
     return w;
   }
@@ -557,17 +575,27 @@ struct WideLayout {
   }
 
   __device__ __forceinline__ double mv() const {
-    const double* m = sM;
     const double2* z = vec;
     const int i = tid;
-    return row_dot4<N, kChunk>([&](int j) { return m[i + j * kLd]; },
+    if (kLdsM) {
+      const double* m = sM;
+      return row_dot4<N, kChunk>([&](int j) { return m[i + j * kLd]; },
+                                 [&](int j) { return z[j].x; });
+    }
+    const double* m = gM;
+    return row_dot4<N, kChunk>([&](int j) { return m[i + (size_t)j * N]; },
                                [&](int j) { return z[j].x; });
   }
   __device__ __forceinline__ double mtv() const {
-    const double* m = sM;
     const double2* z = vec;
     const int i = tid;
-    return row_dot4<N, kChunk>([&](int j) { return m[j + i * kLd]; },
+    if (kLdsM) {
+      const double* m = sM;
+      return row_dot4<N, kChunk>([&](int j) { return m[j + i * kLd]; },
+                                 [&](int j) { return z[j].y; });
+    }
+    const double* mt = gMT;
+    return row_dot4<N, kChunk>([&](int j) { return mt[i + (size_t)j * N]; },
                                [&](int j) { return z[j].y; });
   }
 

@@ -287,3 +287,19 @@ def config2_class(n, ns=4096, seed=2025, periods=64):
 def config2(ns_per_class=4096, seed=2025, units=CONFIG2_UNITS):
     """Config 2: 5 x 4096 systems with N in {8, 16, 32, 64, 128}, one Ensemble per size."""
     return [config2_class(n, ns_per_class, seed) for n in units]
+
+
+def config4(ns=4096, seed=2026, periods=256, n=256):
+    """Config 4: ns systems of n = 256 bubbles at random 3D positions in a sphere of 1 mm
+    diameter with at least 30 um separation (per-system geometry and coupling), radii
+    log-uniform 2-8 um, forcing 2-3 bar at 25-50 kHz drawn per system."""
+    rng = np.random.default_rng([seed, n])
+    _, d = random_positions_batch(ns, n, 30.0, rng, sphere_radius_um=500.0)
+    re = np.exp(rng.uniform(math.log(2.0), math.log(8.0), size=(ns, n)))
+    pa = rng.uniform(2.0, 3.0, size=ns)
+    f = rng.uniform(25.0, 50.0, size=ns)
+    params = unit_parameters(pa[:, None], f[:, None], re)
+    M = coupling_from_geometry(re, d)
+    state0 = np.zeros((ns, n, 2))
+    state0[..., 0] = 1.0
+    return Ensemble(f"cfg4-N{n}", n, pa, f, re, params, M, state0, default_options(), periods)

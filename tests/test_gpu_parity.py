@@ -29,6 +29,17 @@ def test_small_ensembles_bitexact(oracle, N, layout):
     s.close()
 
 
+@pytest.mark.parametrize("N,end", [(128, 0.2), (256, 0.05)])
+def test_wide_layouts_bitexact(oracle, N, end):
+    """Shared coupling matrix, multi-wave systems: N = 128 (M in LDS), N = 256 (M streamed)."""
+    ens = wl.small_ensemble(n=N, ns=3, seed=3 + N)
+    print(f"small ensemble N={N} NS={ens.NS} end={end}", flush=True)
+    s = run_gpu(ens, periods=1, end_time=end)
+    p = run_oracle(oracle, ens, periods=1, end_time=end)
+    compare_solver_to_oracle(s, p)
+    s.close()
+
+
 @pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
 def test_rk4_bitexact(oracle, layout):
     ens = wl.small_ensemble(n=32, ns=4, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))

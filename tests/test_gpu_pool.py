@@ -141,3 +141,15 @@ def test_pool_autotune_restores_state():
         assert_bitwise(f"class {c} state after autotune", s.host_view(V.ActualState), before[c])
         assert not s.stats().any()
     pool.close()
+
+
+def test_config4_class_matches_oracle(oracle):
+    """Config-4 shape: N = 256 clusters in a 1 mm sphere, per-system coupling streamed from
+    global memory; first 2 % of a period against the oracle, system by system."""
+    classes = [wl.config4(ns=2, seed=5)]
+    print("case: config 4 class, N=256, NS=2, end 0.02", flush=True)
+    pool = _pool(classes, end=0.02)
+    pool.Solve()
+    res = _results(pool, classes)
+    pool.close()
+    _check_against_oracle(oracle, classes, res, 1, 0.02)
