@@ -18,6 +18,7 @@
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <vector>
 #include <type_traits>
 
 #include "bex_capi.h"
@@ -65,7 +66,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   BiCGResidualTolerance,
   BiCGMaxIterations,
   KernelLayout,
-  PriorityStepThreshold
+  PriorityStepThreshold,
+  WarmStart
 };
 
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,
@@ -240,6 +242,12 @@ class ProblemSolver {
   // Additions: the underlying handle and the per-system counters.
   bex_solver* Handle() { return handle_; }
 
+  // Learned BiCG start (no reference counterpart): 4609 FP32 parameters of the network in
+  // bex_mlp.h; enable with SolverOption(WarmStart, 1).
+  void SetWarmStartNetwork(const float* params) {
+    Check(bex_set_warm_start_network(handle_, params), "SetWarmStartNetwork");
+  }
+
   static const char* VariableName(ListOfVariables v) {
     static const char* names[] = {"All",
                                   "TimeDomain",
@@ -262,6 +270,60 @@ class ProblemSolver {
                                   "DenseState"};
     return names[(int)v];
   }
+};
+
+// Uneven ensembles (no reference counterpart): size classes of systems with their own
+// coupling matrices, one graph launch per period. Class(c) is a solver handle for the C calls
+// (bex_set_host_*, bex_get_host_*, bex_solver_option*, bex_sync_*).
+class ProblemPool {
+ private:
+  bex_pool* pool_ = nullptr;
+
+  void Check(int rc, const char* function) const {
+    if (rc == BEX_OK) return;
+    std::cerr << "ERROR: In pool member function " << function << "!" << std::endl;
+    std::cerr << "       " << bex_pool_last_error(pool_) << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+
+ public:
+  ProblemPool(const std::vector<int>& units, const std::vector<int>& systems, int device) {
+    if (units.size() != systems.size()) {
+      std::cerr << "ERROR: ProblemPool: units and systems differ in length" << std::endl;
+      std::exit(EXIT_FAILURE);
+    }
+    if (bex_pool_create((int)units.size(), units.data(), systems.data(), device, &pool_) !=
+        BEX_OK) {
+      std::cerr << "ERROR: creating the pool failed!" << std::endl;
+      std::cerr << "       " << bex_pool_last_error(nullptr) << std::endl;
+      std::exit(EXIT_FAILURE);
+    }
+  }
+  ~ProblemPool() {
+    if (pool_) bex_pool_destroy(pool_);
+  }
+  ProblemPool(const ProblemPool&) = delete;
+  ProblemPool& operator=(const ProblemPool&) = delete;
+
+  bex_solver* Class(int c) {
+    bex_solver* s = nullptr;
+    Check(bex_pool_class_solver(pool_, c, &s), "Class");
+    return s;
+  }
+  void SetCouplings(int c, int first, int count, const double* M) {
+    Check(bex_pool_set_couplings(pool_, c, first, count, M), "SetCouplings");
+  }
+  void SetOrder(int c, const std::vector<int>& order) {
+    Check(bex_pool_set_order(pool_, c, order.data()), "SetOrder");
+  }
+  void OrderByCost(int c) { Check(bex_pool_order_by_cost(pool_, c), "OrderByCost"); }
+  void Autotune(double probe_fraction) {
+    Check(bex_pool_autotune(pool_, probe_fraction, nullptr), "Autotune");
+  }
+  void Solve() { Check(bex_pool_solve(pool_), "Solve"); }
+  void SolvePeriods(int periods) { Check(bex_pool_solve_periods(pool_, periods), "SolvePeriods"); }
+  void Synchronise() { Check(bex_pool_synchronise(pool_), "Synchronise"); }
+  bex_pool* Handle() { return pool_; }
 };
 
 #endif
