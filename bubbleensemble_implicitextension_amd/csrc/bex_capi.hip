@@ -71,6 +71,8 @@ struct bex_solver {
   unsigned long long* d_sample_count = nullptr;
   long long SampleCapacity = 0;
   int SampleEvery = 0;
+  int DenseInterpolation = 0;         // 1: Hermite dense output at a uniform grid
+  double* d_DenseAux = nullptr;       // [2 UD][TLT], allocated with the option
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
   int config_epoch = 0;  // bumped by every option change (pools re-capture their graph)
   std::string err;
@@ -212,6 +214,8 @@ void fill_args(const bex_solver* s, BexKernelArgs* ap) {
   a.SampleCount = s->d_sample_count;
   a.SampleCapacity = s->SampleCapacity;
   a.SampleEvery = s->SampleEvery > 0 ? s->SampleEvery : 1;
+  a.DenseInterpolation = (s->DenseInterpolation && s->d.NDO > 0 && s->d_DenseAux) ? 1 : 0;
+  a.DenseAux = s->d_DenseAux;
   a.ref_spb = s->d.SPB;
   a.ref_block_stride = s->LogicalThreadsPerBlock + s->ThreadPaddingPerBlock;
 }
@@ -505,6 +509,7 @@ int bex_destroy(bex_solver* s) {
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
   if (s->d_timeline) hipFree(s->d_timeline);
+  if (s->d_DenseAux) hipFree(s->d_DenseAux);
   if (s->d_net) hipFree(s->d_net);
   if (s->d_samples) hipFree(s->d_samples);
   if (s->d_sample_count) hipFree(s->d_sample_count);
@@ -528,6 +533,14 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_TimeStepShrinkLimit: s->TimeStepShrinkLimit = v; break;
     case BEX_DenseOutputMinimumTimeStep: s->DenseOutputMinimumTimeStep = v; break;
     case BEX_DenseOutputSaveFrequency: s->DenseOutputSaveFrequency = (int)v; break;
+    case BEX_DenseOutputInterpolation: {
+      if (v != 0.0 && !s->d_DenseAux && s->d.NDO > 0) {
+        BEX_HIP(s, hipSetDevice(s->device));
+        BEX_HIP(s, hipMalloc((void**)&s->d_DenseAux, sizeof(double) * 2 * s->d.UD * (size_t)s->TLT));
+      }
+      s->DenseInterpolation = (v != 0.0) ? 1 : 0;
+      break;
+    }
     case BEX_SharedGlobalVariables: s->SharedGlobalVariables = (v != 0.0); break;
     case BEX_SharedCouplingMatrices: s->SharedCouplingMatrices = (v != 0.0); break;
     case BEX_BiCGRhoTolerance: s->BiCGRhoTolerance = v; break;
@@ -852,6 +865,9 @@ int bex_solve(bex_solver* s) {
     return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count, layout and mode");
   BexKernelArgs a{};
   fill_args(s, &a);
+  if (a.DenseInterpolation && !(s->DenseOutputMinimumTimeStep > 0.0))
+    return fail(s, BEX_E_OPTION,
+                "Solve: interpolated dense output needs DenseOutputMinimumTimeStep > 0");
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
   if (nsys <= 0) return BEX_OK;
   if (kc.lds > 65536)

@@ -1011,6 +1011,47 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     store_dense(0);
   }
 
+  // Interpolated dense output (no reference counterpart): outputs at t0 + k * DenseOutput-
+  // MinimumTimeStep, each by cubic Hermite interpolation over the accepted step that contains it,
+  // from the step's end points and their derivatives. The end point's derivative is the next
+  // attempt's first stage, so an interval is flushed at that stage; after the last step one
+  // extra first-stage evaluation flushes the final interval. The step's start point and
+  // derivative wait in DenseAux ([2 UD][TLT]).
+  const bool interp = (a.NDO > 0) && (a.DenseInterpolation != 0);
+  bool pending = false, flushOnly = false;
+  double tA = 0.0, tB = 0.0;
+  const double gridT0 = t;
+  long long kq = 1;
+  auto dense_flush = [&](const double* fB) {
+    for (;;) {
+      const int idx = S.DOIDX;
+      if (idx >= a.NDO) break;
+      const double tq = gridT0 + (double)kq * a.DenseOutputMinimumTimeStep;
+      if (!(tq <= tB)) break;
+      const double h = tB - tA;
+      const double th = (tq - tA) / h;
+      const double sm = 1.0 - th;
+      const double h00 = ((1.0 + 2.0 * th) * sm) * sm;
+      const double h10 = (th * sm) * sm;
+      const double h01 = (th * th) * (3.0 - 2.0 * th);
+      const double h11 = (th * th) * (th - 1.0);
+      if (primal) {
+#pragma unroll
+        for (int i = 0; i < UD; i++) {
+          const double yA = a.DenseAux[gtid + i * TLT];
+          const double fA = a.DenseAux[gtid + (UD + i) * TLT];
+          const double y = ((h00 * yA + (h10 * h) * fA) + h01 * x[i]) + (h11 * h) * fB[i];
+          a.DenseOutputStates[gtid + i * TLT + (long long)idx * TLT * UD] = y;
+        }
+        if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = tq;
+      }
+      lay.sync();
+      if (uid == 0) S.DOIDX = idx + 1;
+      lay.sync();
+      kq++;
+    }
+  };
+
   int terminate = 0;
   int accepted = 0;
   long long stage_count = 0;  // RK stages of this launch (sample selection)
@@ -1021,7 +1062,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       dt = TD1 - t;
       endReached = 1;
     }
-    if (leader) st[BEX_STAT_ATTEMPTS]++;
+    if (leader && !flushOnly) st[BEX_STAT_ATTEMPTS]++;
 
     bool finiteLane = true;
     double k1[UD], k2[UD], k3[UD], k4[UD], k5[UD], k6[UD];
@@ -1110,6 +1151,15 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
           default: k6[i] = F[i]; break;
         }
       }
+      if (interp && s == 0 && pending) {
+        dense_flush(F);
+        pending = false;
+      }
+      if (flushOnly) break;
+    }
+    if (flushOnly) {
+      terminate = 1;
+      continue;
     }
 
     if (ALG == kRKCK45) {
@@ -1208,6 +1258,16 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
 
     if (updateStep == 1) {  // Solver.cuh:433-525
+      if (interp) {
+        if (primal) {
+#pragma unroll
+          for (int i = 0; i < UD; i++) {
+            a.DenseAux[gtid + i * TLT] = x[i];
+            a.DenseAux[gtid + (UD + i) * TLT] = k1[i];
+          }
+        }
+        tA = t;
+      }
       t += dt;
 #pragma unroll
       for (int i = 0; i < UD; i++) x[i] = xn[i];
@@ -1249,12 +1309,16 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       }
       lay.sync();
       t = S.T;
+      if (interp) {
+        tB = t;
+        pending = true;
+      }
     } else if (leader) {
       st[BEX_STAT_REJECTED]++;
     }
     const int udt = S.UDT;
 
-    if (a.NDO > 0) {  // Solver.cuh:528-557
+    if (a.NDO > 0 && !interp) {  // Solver.cuh:528-557
       if (updateStep == 1) {
         const int idx = S.DOIDX;
         updateDO = ((idx < a.NDO) && (doTime < t) && (skipped >= (a.DenseOutputSaveFrequency - 1)))
@@ -1267,7 +1331,12 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       store_dense(updateStep);
     }
 
-    if (updateStep == 1 && (endReached == 1 || udt == 1)) terminate = 1;  // :560-569
+    if (updateStep == 1 && (endReached == 1 || udt == 1)) {  // :560-569
+      if (interp && pending && S.DOIDX < a.NDO)
+        flushOnly = true;  // one more first-stage evaluation, then stop
+      else
+        terminate = 1;
+    }
   }
 
   // Finalization hook and write back (Solver.cuh:577-647).

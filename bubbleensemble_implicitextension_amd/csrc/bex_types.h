@@ -75,4 +75,8 @@ struct BexKernelArgs {
   unsigned long long* SampleCount;   // rows reserved so far (may exceed the capacity)
   long long SampleCapacity;
   int SampleEvery;                   // a system records every SampleEvery-th RK stage
+
+  // Interpolated dense output (no reference counterpart).
+  int DenseInterpolation;  // 0: step-point sampling (reference), 1: Hermite at a uniform grid
+  double* DenseAux;        // [2 UD][TLT]: start state and derivative of the pending step
 };

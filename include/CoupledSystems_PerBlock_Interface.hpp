@@ -67,7 +67,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   BiCGMaxIterations,
   KernelLayout,
   PriorityStepThreshold,
-  WarmStart
+  WarmStart,
+  DenseOutputInterpolation
 };
 
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,

@@ -87,7 +87,12 @@ enum bex_option {
   /* 1: BiCG starts from the learned guess x0 = b * y(features) instead of the uncoupled
    * acceleration (Steppers.cuh:494-499), with a restart from the uncoupled acceleration when
    * that solve does not converge; needs bex_set_warm_start_network. 32 units only. */
-  BEX_WarmStart
+  BEX_WarmStart,
+  /* 1: dense output at t0 + k * DenseOutputMinimumTimeStep by cubic Hermite interpolation over
+   * the accepted step containing each instant (from its end points and their derivatives),
+   * instead of the reference's step-point sampling (DenseOutput.cuh:313-395); the index still
+   * persists across Solve() calls. No reference counterpart. */
+  BEX_DenseOutputInterpolation
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32 units, lane groups otherwise */

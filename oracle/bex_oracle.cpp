@@ -449,6 +449,41 @@ void solve_system(const Ctx& c, int sid) {
     store_dense(0);
   }
 
+  // Interpolated dense output (no reference counterpart; the kernel's DenseInterpolation):
+  // outputs at t0 + k * DenseOutputMinimumTimeStep by cubic Hermite interpolation over the
+  // accepted step containing them; the step's end-point derivative is the next attempt's first
+  // stage, and one extra first-stage evaluation flushes the last step.
+  const bool interp = (d.NDO > 0) && (o.DenseOutputInterpolation != 0);
+  bool pending = false, flushOnly = false;
+  double tA = 0.0, tB = 0.0;
+  const double gridT0 = t;
+  long long kq = 1;
+  std::vector<double> YA(N * UD), FA(N * UD);
+  auto dense_flush = [&](const double* K0) {
+    while (doIdx < d.NDO) {
+      const double tq = gridT0 + (double)kq * o.DenseOutputMinimumTimeStep;
+      if (!(tq <= tB)) break;
+      const double h = tB - tA;
+      const double th = (tq - tA) / h;
+      const double sm = 1.0 - th;
+      const double h00 = ((1.0 + 2.0 * th) * sm) * sm;
+      const double h10 = (th * sm) * sm;
+      const double h01 = (th * th) * (3.0 - 2.0 * th);
+      const double h11 = (th * th) * (th - 1.0);
+      for (int u = 0; u < N; u++) {
+        long g = (long)sid * N + u;
+        for (int i = 0; i < UD; i++) {
+          const int q = u * UD + i;
+          const double y = ((h00 * YA[q] + (h10 * h) * FA[q]) + h01 * X[q]) + (h11 * h) * K0[q];
+          b.DenseOutputStates[g + i * TLT + (long)doIdx * TLT * UD] = y;
+        }
+      }
+      b.DenseOutputTimeInstances[sid + (long)doIdx * NS] = tq;
+      doIdx++;
+      kq++;
+    }
+  };
+
   auto stage = [&](int s, double Ts) {  // Steppers.cuh:431-562 for all units
     double* Ks = &K[s * N * UD];
     for (int u = 0; u < N; u++) {
@@ -465,10 +500,20 @@ void solve_system(const Ctx& c, int sid) {
     st[BEX_STAT_BICG_RHO_EXITS] += br.rho_exit ? 1 : 0;
     st[BEX_STAT_BICG_MAXIT] += br.maxit ? 1 : 0;
     st[BEX_STAT_HIST0 + (br.iterations < 20 ? br.iterations : 19)]++;
+    if (interp && s == 0 && pending) {
+      dense_flush(Ks);
+      pending = false;
+    }
   };
 
   // Time loop (Solver.cuh:329-572)
   while (!terminate) {
+    if (flushOnly) {  // first stage at the final point, then stop
+      for (int i = 0; i < N * UD; i++) Xs[i] = X[i];
+      stage(0, t);
+      terminate = 1;
+      break;
+    }
     int updateStep = 1, isFinite = 1, endReached = 0;  // :334-344
     dt = dtNew;
     if (dt > (TD[1] - t)) {
@@ -546,6 +591,13 @@ void solve_system(const Ctx& c, int sid) {
 
     // Update (Solver.cuh:433-441)
     if (updateStep == 1) {
+      if (interp) {
+        for (int i = 0; i < n2; i++) {
+          YA[i] = X[i];
+          FA[i] = K[i];  // k1 of the accepted step
+        }
+        tA = t;
+      }
       t += dt;
       for (int i = 0; i < n2; i++) X[i] = Xn[i];
       st[BEX_STAT_ACCEPTED]++;
@@ -568,12 +620,16 @@ void solve_system(const Ctx& c, int sid) {
         for (int u = 0; u < N; u++) km_event(&EN[u * d.NE], &X[u * UD]);
         for (int k = 0; k < N * d.NE; k++) EA[k] = EN[k];
       }
+      if (interp) {
+        tB = t;
+        pending = true;
+      }
     } else {
       st[BEX_STAT_REJECTED]++;
     }
 
     // Dense output (Solver.cuh:528-557; DenseOutput.cuh:368-395)
-    if (d.NDO > 0) {
+    if (d.NDO > 0 && !interp) {
       if (updateStep == 1) {
         updateDO = ((doIdx < d.NDO) && (doTime < t) &&
                     (skipped >= (o.DenseOutputSaveFrequency - 1)))
@@ -587,7 +643,12 @@ void solve_system(const Ctx& c, int sid) {
     }
 
     // Termination (Solver.cuh:560-569)
-    if (updateStep == 1 && (endReached == 1 || udt == 1)) terminate = 1;
+    if (updateStep == 1 && (endReached == 1 || udt == 1)) {
+      if (interp && pending && doIdx < d.NDO)
+        flushOnly = true;
+      else
+        terminate = 1;
+    }
   }
 
   // Write back (Solver.cuh:599-647)

@@ -67,6 +67,7 @@ typedef struct {
   const double* EventTolerance;    /* [NE] */
   const int* EventDirection;       /* [NE] */
   int Algorithm;                   /* 0 = RK4 (fixed step), 1 = RKCK45 */
+  int DenseOutputInterpolation;    /* 0 = step-point sampling (reference), 1 = Hermite grid */
 } bex_oracle_options;
 
 typedef struct {

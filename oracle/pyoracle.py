@@ -49,6 +49,7 @@ class Options(ctypes.Structure):
         ("EventTolerance", ctypes.POINTER(ctypes.c_double)),
         ("EventDirection", ctypes.POINTER(ctypes.c_int)),
         ("Algorithm", ctypes.c_int),
+        ("DenseOutputInterpolation", ctypes.c_int),
     ]
 
 
@@ -184,7 +185,7 @@ class OracleProblem:
                       o["DenseOutputMinimumTimeStep"], o["BiCGRhoTolerance"],
                       o["BiCGResidualTolerance"], int(o["BiCGMaxIterations"]),
                       _dp(self._rt), _dp(self._at), _dp(self._et), _ip(self._ed),
-                      int(o.get("Algorithm", 1)))
+                      int(o.get("Algorithm", 1)), int(o.get("DenseOutputInterpolation", 0)))
         buf = Buffers(_dp(self.TimeDomain), _dp(self.ActualState), _dp(self.ActualTime),
                       _dp(self.UnitParameters), _dp(self.SystemParameters),
                       _dp(self.GlobalParameters), _ip(self.IntegerGlobalParameters),

@@ -92,3 +92,29 @@ def test_config0_within_config1_batch(oracle):
     np.testing.assert_array_equal(acc.view(np.int64),
                                   p.UnitAccessories.reshape(-1, len(keep), N).view(np.int64))
     np.testing.assert_array_equal(s.stats()[sids[keep]], p.stats_table())
+
+
+@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
+@pytest.mark.parametrize("interp", [0, 1])
+def test_dense_output_bitexact(oracle, layout, interp):
+    """Dense output, step-point sampling (reference) and Hermite grid, two periods so that the
+    persistent index carries across Solve() calls."""
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    ens = wl.small_ensemble(n=32, ns=4, seed=9)
+    ndo = 100
+    opts = {"DenseOutputMinimumTimeStep": 1.0 / 40.0, "DenseOutputInterpolation": interp}
+    print(f"dense output N=32 layout={layout} interp={interp}", flush=True)
+    s = run_gpu(ens, periods=2, options=opts, ndo=ndo, layout=layout)
+    p = run_oracle(oracle, ens, periods=2, options=opts, ndo=ndo)
+    compare_solver_to_oracle(s, p)
+    from gpu_helpers import assert_bitwise
+    assert_bitwise("DenseIndex", s.host_view(V.DenseIndex), p.DenseOutputIndex)
+    n = int(p.DenseOutputIndex.max())
+    assert n > 40
+    assert_bitwise("DenseTime", s.host_view(V.DenseTime)[: n * ens.NS],
+                   p.DenseOutputTimeInstances[: n * ens.NS])
+    TLT = ens.NS * ens.N
+    assert_bitwise("DenseState", s.host_view(V.DenseState)[: n * TLT * 2],
+                   p.DenseOutputStates[: n * TLT * 2])
+    s.close()

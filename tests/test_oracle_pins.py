@@ -368,3 +368,51 @@ def test_reference_fixture_coupling_convention():
     assert d[off].min() > 150.0
     rebuilt = wl.coupling_from_geometry(R, d)
     np.testing.assert_allclose(rebuilt[off], M[off], rtol=1e-12)
+
+
+def _dense_problem(oracle_mod, ens, ndo, dt_out, interp, end=1.0, max_step=None):
+    p = oracle_mod.OracleProblem.from_ensemble(ens, NDO=ndo)
+    p.TimeDomain[ens.NS:] = end
+    p.opts["DenseOutputMinimumTimeStep"] = dt_out
+    p.opts["DenseOutputInterpolation"] = interp
+    if max_step is not None:
+        p.opts["MaximumTimeStep"] = max_step
+    p.solve()
+    return p
+
+
+def test_hermite_dense_output_against_integration(oracle):
+    """Interpolated dense output at t_k = k/32 against the integrator's own state at t_k
+    (the same problem solved on [0, t_k]); linear interpolation between the step points, the
+    obvious mistake, is much further off."""
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    ens = wl.small_ensemble(n=8, ns=2, seed=21, pa=(0.6, 0.9), f=(60.0, 90.0))
+    NS, N = ens.NS, ens.N
+    ndo, dt_out = 40, 1.0 / 32.0
+    p = _dense_problem(oracle, ens, ndo, dt_out, 1)
+    idx = p.DenseOutputIndex
+    assert np.all(idx == 33)  # t = 0, 1/32, ..., 1
+    times = p.DenseOutputTimeInstances.reshape(ndo, NS)
+    np.testing.assert_array_equal(times[:33, 0], np.arange(33) / 32.0)
+    dense = p.DenseOutputStates.reshape(ndo, 2, NS, N)
+    # Step points of the same run: the step-point sampler with no minimum spacing.
+    sp = _dense_problem(oracle, ens, 20000, 0.0, 0)
+    spn = sp.DenseOutputIndex
+    spt = sp.DenseOutputTimeInstances.reshape(20000, NS)
+    sps = sp.DenseOutputStates.reshape(20000, 2, NS, N)
+    worst_h, worst_lin = 0.0, 0.0
+    for k in (5, 11, 17, 23, 29):
+        tk = k / 32.0
+        truth = _dense_problem(oracle, ens, 0, 0.0, 0, end=tk).state()  # [NS, N, 2]
+        for s in range(NS):
+            h = dense[k, 0, s, :]
+            tr = truth[s, :, 0]
+            worst_h = max(worst_h, float(np.max(np.abs(h - tr) / np.abs(tr))))
+            ts = spt[: spn[s], s]
+            j = np.searchsorted(ts, tk) - 1
+            w = (tk - ts[j]) / (ts[j + 1] - ts[j])
+            lin = (1 - w) * sps[j, 0, s, :] + w * sps[j + 1, 0, s, :]
+            worst_lin = max(worst_lin, float(np.max(np.abs(lin - tr) / np.abs(tr))))
+    assert worst_h < 1e-7, worst_h
+    assert worst_lin > 20 * worst_h, (worst_lin, worst_h)
