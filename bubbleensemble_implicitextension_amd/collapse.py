@@ -105,3 +105,24 @@ def make_pool(ensembles, device=0):
         M = e.coupling if e.coupling.ndim == 3 else np.broadcast_to(e.coupling, (e.NS, e.N, e.N))
         pool.set_couplings(c, M)
     return pool
+
+
+def stream_dense_output(solver, periods, sink):
+    """Runs `periods` Solve() calls and hands each period's dense output to
+    sink(period, times [n, NS], states [n, UD, NS*UPS]) before the next period reuses the
+    buffers (the dense index is reset to 0 after each period). With interpolation on
+    (DenseOutputInterpolation = 1) every period yields the same uniform instants."""
+    NS, UPS, UD = solver.NS, solver.UPS, solver.UD
+    TLT = NS * UPS
+    for k in range(periods):
+        solver.Solve()
+        solver.SynchroniseFromDeviceToHost(V.DenseOutput)
+        solver.InsertSynchronisationPoint()
+        solver.SynchroniseSolver()
+        idx = solver.host_view(V.DenseIndex)
+        n = int(idx.max()) if idx.size else 0
+        times = solver.host_view(V.DenseTime)[: n * NS].reshape(n, NS).copy()
+        states = solver.host_view(V.DenseState)[: n * UD * TLT].reshape(n, UD, TLT).copy()
+        sink(k, times, states)
+        idx[:] = 0
+        solver.SynchroniseFromHostToDevice(V.DenseIndex)

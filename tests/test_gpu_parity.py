@@ -118,3 +118,30 @@ def test_dense_output_bitexact(oracle, layout, interp):
     assert_bitwise("DenseState", s.host_view(V.DenseState)[: n * TLT * 2],
                    p.DenseOutputStates[: n * TLT * 2])
     s.close()
+
+
+def test_dense_output_streaming_per_period(oracle):
+    """Per-period streaming (index reset after each period) delivers, period by period, the
+    same interpolated instants and states as one persistent-index run."""
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    ens = wl.small_ensemble(n=32, ns=4, seed=9)
+    opts = {"DenseOutputMinimumTimeStep": 1.0 / 40.0, "DenseOutputInterpolation": 1}
+    print("dense output streaming, 2 periods", flush=True)
+    ref = run_gpu(ens, periods=2, options=opts, ndo=100)
+    rt = ref.host_view(V.DenseTime).reshape(100, ens.NS)
+    rs = ref.host_view(V.DenseState).reshape(100, 2, ens.NS * ens.N)
+    s = collapse.make_solver(ens, NDO=41)
+    collapse.fill(s, ens)
+    collapse.apply_options(s, opts)
+    s.SynchroniseFromHostToDevice(V.All)
+    got = []
+    collapse.stream_dense_output(s, 2, lambda k, t, x: got.append((t, x)))
+    assert [g[0].shape[0] for g in got] == [41, 41]
+    from gpu_helpers import assert_bitwise
+    for k, (t, x) in enumerate(got):
+        assert_bitwise(f"period {k} times", t, rt[41 * k: 41 * (k + 1)])
+        assert_bitwise(f"period {k} states", x, rs[41 * k: 41 * (k + 1)])
+    s.close()
+    ref.close()
