@@ -125,4 +125,4 @@ def stream_dense_output(solver, periods, sink):
         states = solver.host_view(V.DenseState)[: n * UD * TLT].reshape(n, UD, TLT).copy()
         sink(k, times, states)
         idx[:] = 0
-        solver.SynchroniseFromHostToDevice(V.DenseIndex)
+        solver.SynchroniseFromHostToDevice(V.DenseOutput)  # index, times and states

@@ -117,33 +117,40 @@ struct KernelChoice {
   int block;
 };
 
+// Dense output compiles into separate kernels, so launches without it (NDO = 0) carry none of
+// its registers.
 template <class L>
-KernelChoice choice(int alg) {
-  KernelFn fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4>
-                               : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45>;
+KernelChoice choice(int alg, bool dense) {
+  KernelFn fn;
+  if (dense)
+    fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4, true>
+                        : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45, true>;
+  else
+    fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4, false>
+                        : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45, false>;
   return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
 }
 
 // layout: BEX_LAYOUT_AUTO picks the role-pair layout for 32 units and the group layout
 // otherwise. mode: bexk::WarmStartMode; the learned guess and sample collection exist for the
 // role-pair layout only.
-bool kernel_for(int ups, int alg, int layout, int mode, KernelChoice* k) {
+bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice* k) {
   if (ups == 32 && layout != BEX_LAYOUT_GROUP) {
     switch (mode) {
-      case bexk::kPlain: *k = choice<bexk::RolePairLayout32<bexk::kPlain>>(alg); return true;
-      case bexk::kSampling: *k = choice<bexk::RolePairLayout32<bexk::kSampling>>(alg); return true;
-      case bexk::kLearned: *k = choice<bexk::RolePairLayout32<bexk::kLearned>>(alg); return true;
+      case bexk::kPlain: *k = choice<bexk::RolePairLayout32<bexk::kPlain>>(alg, dense); return true;
+      case bexk::kSampling: *k = choice<bexk::RolePairLayout32<bexk::kSampling>>(alg, dense); return true;
+      case bexk::kLearned: *k = choice<bexk::RolePairLayout32<bexk::kLearned>>(alg, dense); return true;
       default: return false;
     }
   }
   if (layout == BEX_LAYOUT_ROLE_PAIR || mode != bexk::kPlain) return false;
   switch (ups) {
-    case 8: *k = choice<bexk::GroupLayout<8>>(alg); return true;
-    case 16: *k = choice<bexk::GroupLayout<16>>(alg); return true;
-    case 32: *k = choice<bexk::GroupLayout<32>>(alg); return true;
-    case 64: *k = choice<bexk::GroupLayout<64>>(alg); return true;
-    case 128: *k = choice<bexk::WideLayout<128, false>>(alg); return true;
-    case 256: *k = choice<bexk::WideLayout<256, false>>(alg); return true;
+    case 8: *k = choice<bexk::GroupLayout<8>>(alg, dense); return true;
+    case 16: *k = choice<bexk::GroupLayout<16>>(alg, dense); return true;
+    case 32: *k = choice<bexk::GroupLayout<32>>(alg, dense); return true;
+    case 64: *k = choice<bexk::GroupLayout<64>>(alg, dense); return true;
+    case 128: *k = choice<bexk::WideLayout<128, false>>(alg, dense); return true;
+    case 256: *k = choice<bexk::WideLayout<256, false>>(alg, dense); return true;
     default: return false;
   }
 }
@@ -370,7 +377,7 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
       (d.CCI != 0 && d.CCI != 1) || (d.Algorithm != BEX_RK4 && d.Algorithm != BEX_RKCK45))
     return fail(nullptr, BEX_E_ARGUMENT, "bex_create: invalid problem description");
   KernelChoice kc;
-  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, &kc))
+  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, false, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
                 "bex_create: units per system must be 8, 16, 32, 64, 128 or 256 in this build");
 
@@ -549,14 +556,14 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_PriorityStepThreshold: s->PriorityStepThreshold = (int)v; break;
     case BEX_KernelLayout: {
       KernelChoice kc;
-      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, &kc))
+      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, false, &kc))
         return fail(s, BEX_E_UNSUPPORTED, "SolverOption: no kernel layout for this unit count");
       s->KernelLayout = (int)v;
       break;
     }
     case BEX_WarmStart: {
       KernelChoice kc;
-      if (v != 0.0 && !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kLearned, &kc))
+      if (v != 0.0 && !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kLearned, false, &kc))
         return fail(s, BEX_E_UNSUPPORTED,
                     "SolverOption: the learned warm start needs 32 units and the role-pair layout");
       s->WarmStart = (v != 0.0) ? 1 : 0;
@@ -861,7 +868,7 @@ int bex_solve(bex_solver* s) {
   const int mode = s->WarmStart ? bexk::kLearned : (s->d_samples ? bexk::kSampling : bexk::kPlain);
   if (mode == bexk::kLearned && !s->d_net)
     return fail(s, BEX_E_ARGUMENT, "Solve: warm start enabled but no network was set");
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, &kc))
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, s->d.NDO > 0, &kc))
     return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count, layout and mode");
   BexKernelArgs a{};
   fill_args(s, &a);
@@ -1009,7 +1016,7 @@ int bex_enable_sampling(bex_solver* s, long long capacity, int every) {
   s->SampleEvery = 0;
   if (capacity == 0) return BEX_OK;
   KernelChoice kc;
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kSampling, &kc))
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kSampling, false, &kc))
     return fail(s, BEX_E_UNSUPPORTED,
                 "bex_enable_sampling: sample collection needs 32 units and the role-pair layout");
   BEX_HIP(s, hipMalloc((void**)&s->d_samples, sizeof(float) * BEX_SAMPLE_FLOATS * (size_t)capacity));

@@ -68,6 +68,10 @@ struct SysShared {
   int isACC[D::NiSA > 0 ? D::NiSA : 1];
   int UDT;
   int DOIDX;
+  // Interpolated dense output: the pending step's end times, the output grid and flags.
+  double tA, tB, gridT0;
+  long long kq;
+  int pending, flushOnly;
 };
 using Sys = SysShared<BexSystemDims>;
 
@@ -903,7 +907,8 @@ struct RolePairLayout32 {
 // =========================================================================================
 // Time loop of one system (Solver.cuh:11-651), shared by the layouts.
 
-template <class L, int ALG>
+// DENSE = false compiles a kernel without any dense-output code (launches with NDO = 0).
+template <class L, int ALG, bool DENSE>
 __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   using D = BexSystemDims;
   constexpr int N = L::kUnits;
@@ -939,7 +944,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     for (int i = 0; i < D::NSA; i++) S.sACC[i] = a.SystemAccessories[sid + i * NS];
     for (int i = 0; i < D::NiSA; i++) S.isACC[i] = a.IntegerSystemAccessories[sid + i * NS];
     S.UDT = 0;
-    S.DOIDX = (a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
+    S.DOIDX = (DENSE && a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
   }
 
   // Unit scope into registers / private memory (Solver.cuh:234-257).
@@ -1006,7 +1011,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
     if ((updateDO == 0) && (updateStep == 1)) skipped++;
   };
-  if (a.NDO > 0) {
+  if (DENSE && a.NDO > 0) {
     updateDO = (S.DOIDX < a.NDO) ? 1 : 0;
     store_dense(0);
   }
@@ -1017,16 +1022,22 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   // attempt's first stage, so an interval is flushed at that stage; after the last step one
   // extra first-stage evaluation flushes the final interval. The step's start point and
   // derivative wait in DenseAux ([2 UD][TLT]).
-  const bool interp = (a.NDO > 0) && (a.DenseInterpolation != 0);
-  bool pending = false, flushOnly = false;
-  double tA = 0.0, tB = 0.0;
-  const double gridT0 = t;
-  long long kq = 1;
+  // The interval and grid bookkeeping lives in the system's LDS record (uniform values; kept
+  // out of the registers that hold the stage vectors).
+  const bool interp = DENSE && (a.NDO > 0) && (a.DenseInterpolation != 0);
+  if (uid == 0) {
+    S.pending = 0;
+    S.flushOnly = 0;
+    S.gridT0 = t;
+    S.kq = 1;
+  }
+  lay.sync();
   auto dense_flush = [&](const double* fB) {
     for (;;) {
       const int idx = S.DOIDX;
       if (idx >= a.NDO) break;
-      const double tq = gridT0 + (double)kq * a.DenseOutputMinimumTimeStep;
+      const double tA = S.tA, tB = S.tB;
+      const double tq = S.gridT0 + (double)S.kq * a.DenseOutputMinimumTimeStep;
       if (!(tq <= tB)) break;
       const double h = tB - tA;
       const double th = (tq - tA) / h;
@@ -1046,9 +1057,11 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = tq;
       }
       lay.sync();
-      if (uid == 0) S.DOIDX = idx + 1;
+      if (uid == 0) {
+        S.DOIDX = idx + 1;
+        S.kq = S.kq + 1;
+      }
       lay.sync();
-      kq++;
     }
   };
 
@@ -1062,15 +1075,17 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       dt = TD1 - t;
       endReached = 1;
     }
-    if (leader && !flushOnly) st[BEX_STAT_ATTEMPTS]++;
+    if (leader && S.flushOnly == 0) st[BEX_STAT_ATTEMPTS]++;
 
     bool finiteLane = true;
     double k1[UD], k2[UD], k3[UD], k4[UD], k5[UD], k6[UD];
     constexpr int NSTAGE = (ALG == kRKCK45) ? 6 : 4;
+    // The final pass of interpolated dense output evaluates the first stage only.
+    const int nstage = (interp && S.flushOnly != 0) ? 1 : NSTAGE;
     // One copy of the right-hand side + BiCG body: the stage loop is not unrolled and each
     // stage's input state is formed in the reference's association (Steppers.cuh:153-400).
 #pragma nounroll
-    for (int s = 0; s < NSTAGE; s++) {
+    for (int s = 0; s < nstage; s++) {
       double xs[UD], F[UD];
       double Ts;
 #pragma unroll
@@ -1151,13 +1166,14 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
           default: k6[i] = F[i]; break;
         }
       }
-      if (interp && s == 0 && pending) {
-        dense_flush(F);
-        pending = false;
-      }
-      if (flushOnly) break;
     }
-    if (flushOnly) {
+    // k1 = f(t, x) is the pending step's end-point derivative.
+    if (interp && S.pending != 0) {
+      dense_flush(k1);
+      if (uid == 0) S.pending = 0;
+      lay.sync();
+    }
+    if (interp && S.flushOnly != 0) {
       terminate = 1;
       continue;
     }
@@ -1266,7 +1282,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
             a.DenseAux[gtid + (UD + i) * TLT] = k1[i];
           }
         }
-        tA = t;
+        if (uid == 0) S.tA = t;
       }
       t += dt;
 #pragma unroll
@@ -1310,15 +1326,18 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       lay.sync();
       t = S.T;
       if (interp) {
-        tB = t;
-        pending = true;
+        if (uid == 0) {
+          S.tB = t;
+          S.pending = 1;
+        }
+        lay.sync();
       }
     } else if (leader) {
       st[BEX_STAT_REJECTED]++;
     }
     const int udt = S.UDT;
 
-    if (a.NDO > 0 && !interp) {  // Solver.cuh:528-557
+    if (DENSE && a.NDO > 0 && !interp) {  // Solver.cuh:528-557
       if (updateStep == 1) {
         const int idx = S.DOIDX;
         updateDO = ((idx < a.NDO) && (doTime < t) && (skipped >= (a.DenseOutputSaveFrequency - 1)))
@@ -1332,10 +1351,12 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
 
     if (updateStep == 1 && (endReached == 1 || udt == 1)) {  // :560-569
-      if (interp && pending && S.DOIDX < a.NDO)
-        flushOnly = true;  // one more first-stage evaluation, then stop
-      else
+      if (interp && S.pending != 0 && S.DOIDX < a.NDO) {
+        if (uid == 0) S.flushOnly = 1;  // one more first-stage evaluation, then stop
+        lay.sync();
+      } else {
         terminate = 1;
+      }
     }
   }
 
@@ -1365,7 +1386,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     a.TimeDomain[sid + NS] = S.TD[1];
     for (int i = 0; i < D::NSA; i++) a.SystemAccessories[sid + i * NS] = S.sACC[i];
     for (int i = 0; i < D::NiSA; i++) a.IntegerSystemAccessories[sid + i * NS] = S.isACC[i];
-    if (a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
+    if (DENSE && a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
   }
   lay.sync();
   if (primal)
@@ -1373,7 +1394,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   if (leader && a.Timeline) a.Timeline[2 * (long long)sid + 1] = (long long)wall_clock64();
 }
 
-template <class L, int ALG>
+template <class L, int ALG, bool DENSE>
 __global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
 void solve_kernel(const BexKernelArgs a, long long count) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1381,7 +1402,7 @@ void solve_kernel(const BexKernelArgs a, long long count) {
   lay.init(smem, a);
   lay.bind(a, nullptr, (long long)blockIdx.x * L::kItemsPerBlock + threadIdx.x / (L::kBlock / L::kItemsPerBlock),
            count);
-  integrate<L, ALG>(lay, a);
+  integrate<L, ALG, DENSE>(lay, a);
 }
 
 // Persistent variant: a fixed grid whose waves (or workgroups, for multi-wave systems) take
@@ -1405,7 +1426,7 @@ void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long
     lay.sync();
     if (item >= nitems) break;
     lay.bind(a, order, item, count);
-    integrate<L, ALG>(lay, a);
+    integrate<L, ALG, false>(lay, a);  // pools run without dense output
     lay.sync();
   }
 }
