@@ -149,8 +149,8 @@ bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice
     case 16: *k = choice<bexk::GroupLayout<16>>(alg, dense); return true;
     case 32: *k = choice<bexk::GroupLayout<32>>(alg, dense); return true;
     case 64: *k = choice<bexk::GroupLayout<64>>(alg, dense); return true;
-    case 128: *k = choice<bexk::WideLayout<128, false>>(alg, dense); return true;
-    case 256: *k = choice<bexk::WideLayout<256, false>>(alg, dense); return true;
+    case 128: *k = choice<bexk::WideLayout<128, false, 2>>(alg, dense); return true;
+    case 256: *k = choice<bexk::WideLayout<256, false, 2>>(alg, dense); return true;
     default: return false;
   }
 }
@@ -1092,9 +1092,11 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<64, 1, true>>("group64 x1 wave"),
       pool_variant<bexk::GroupLayout<64, 2, true>>("group64 x2 waves")};
   static const std::vector<PoolVariant> v128 = {
-      pool_variant<bexk::WideLayout<128, true>>("wide128 x2 waves")};
+      pool_variant<bexk::WideLayout<128, true, 2>>("wide128 lane pairs x4 waves"),
+      pool_variant<bexk::WideLayout<128, true, 1>>("wide128 x2 waves")};
   static const std::vector<PoolVariant> v256 = {
-      pool_variant<bexk::WideLayout<256, true>>("wide256 x4 waves, M streamed")};
+      pool_variant<bexk::WideLayout<256, true, 2>>("wide256 lane pairs x8 waves, M streamed"),
+      pool_variant<bexk::WideLayout<256, true, 1>>("wide256 x4 waves, M streamed")};
   static const std::vector<PoolVariant> none;
   switch (n) {
     case 8: return v8;

@@ -467,20 +467,53 @@ struct GroupLayout {
   }
 };
 
-// N = 128 and 256: one system per workgroup of N / 64 waves, lane = unit. Reductions combine
-// the waves through LDS (the tree's cross-wave levels, s[i] op s[i + 64] for N = 128 and
-// (s[i] op s[i + 128]) op (s[i + 64] op s[i + 192]) for N = 256) and finish with an in-wave
-// tree that every wave evaluates identically; system-scope synchronisation is the workgroup
-// barrier. N = 128 keeps the system's matrix in LDS (132 KB, one system per CU); a 256 x 256
-// FP64 matrix (512 KB) exceeds the CU's LDS, so N = 256 streams M and M^T (column-major, both
-// products coalesced) from global memory through L2 and MALL.
-template <int N, bool PS = true>
+// Row product split over the two lanes of a unit (lane half h = 0, 1): half h runs the
+// canonical chains j = 2h and j = 2h + 1 (mod 4), each sequential in j; s(2h) + s(2h+1) of the
+// two halves are exchanged with a DPP quad swap and added, giving (s0 + s1) + (s2 + s3) on both
+// lanes (IEEE addition is commutative). C columns per chunk are loaded before their FMAs.
+template <int N, int C, class MF, class ZF>
+__device__ __forceinline__ double row_dot4_pair(MF m, ZF z, int h) {
+  static_assert(C % 4 == 0 && N % C == 0, "chunk of whole column quadruples");
+  double sa = 0.0, sb = 0.0;
+#pragma unroll 1
+  for (int j0 = 0; j0 < N; j0 += C) {
+    double mm[C / 2], zz[C / 2];
+#pragma unroll
+    for (int k = 0; k < C / 2; k++) {
+      const int j = j0 + 4 * (k >> 1) + 2 * h + (k & 1);
+      mm[k] = m(j);
+      zz[k] = z(j);
+    }
+#pragma unroll
+    for (int k = 0; k < C / 2; k += 2) {
+      sa = fma(mm[k], zz[k], sa);
+      sb = fma(mm[k + 1], zz[k + 1], sb);
+    }
+  }
+  const double pair = sa + sb;
+  // quad_perm [1, 0, 3, 2]: the partner lane of the unit
+  const double other = __builtin_amdgcn_mov_dpp(pair, 0xB1, 0xf, 0xf, false);
+  return h == 0 ? pair + other : other + pair;
+}
+
+// N = 128 and 256: one system per workgroup, LPU lanes per unit (N * LPU / 64 waves).
+// Reductions combine the waves through LDS (the tree's cross-wave levels, s[i] op s[i + 64] for
+// N = 128 and (s[i] op s[i + 128]) op (s[i + 64] op s[i + 192]) for N = 256) and finish with
+// an in-wave tree that every wave evaluates identically; system-scope synchronisation is the
+// workgroup barrier. N = 128 keeps the system's matrix in LDS (132 KB, one system per CU); a
+// 256 x 256 FP64 matrix (512 KB) exceeds the CU's LDS, so N = 256 streams M and M^T
+// (column-major, both products coalesced) from global memory through L2 and MALL. With two
+// lanes per unit, each lane runs half of a row's FMA chains (row_dot4_pair) and all four SIMDs
+// of the CU take part; the per-unit work is duplicated on the unit's lanes and the first lane
+// ("primal") writes results.
+template <int N, bool PS = true, int LPU = 1>
 struct WideLayout {
   static_assert(N == 128 || N == 256, "multi-wave layout: 128 or 256 units");
+  static_assert(LPU == 1 || LPU == 2, "one or two lanes per unit");
   static constexpr bool kLdsM = (N == 128);
   static constexpr int kUnits = N;
-  static constexpr int kWaves = N / kWave;
-  static constexpr int kBlock = N;
+  static constexpr int kBlock = N * LPU;
+  static constexpr int kWaves = kBlock / kWave;
   static constexpr int kSystemsPerBlock = 1;
   static constexpr int kWavesPerEU = kLdsM ? 1 : 2;  // N = 128: LDS holds one system per CU
   static constexpr int kChunk = 16;
@@ -495,7 +528,7 @@ struct WideLayout {
            sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
   }
 
-  int tid, sid_;
+  int tid, u, h, sid_;
   double* sM;         // N = 128: padded LDS copy
   const double* gM;   // N = 256: the system's M, column-major, global memory
   const double* gMT;  // N = 256: its transpose
@@ -507,6 +540,8 @@ struct WideLayout {
 
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     tid = threadIdx.x;
+    u = tid / LPU;
+    h = tid % LPU;
     unsigned char* p = smem;
     sM = reinterpret_cast<double*>(p);
     if (kLdsM) p += sizeof(double) * N * (N + 1);
@@ -541,13 +576,13 @@ struct WideLayout {
   }
 
   __device__ __forceinline__ int sid() const { return sid_; }
-  __device__ __forceinline__ int uid() const { return tid; }
-  __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ int uid() const { return u; }
+  __device__ __forceinline__ bool primal() const { return h == 0; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 
   template <class Op>
   __device__ __forceinline__ double reduce(double v, Op op) const {
-    red[tid] = v;
+    if (h == 0) red[u] = v;
     __syncthreads();
     const int l = tid & (kWave - 1);
     const double first = (N == 128) ? op(red[l], red[l + 64])
@@ -578,29 +613,30 @@ struct WideLayout {
     return r;
   }
 
+  template <class MF, class ZF>
+  __device__ __forceinline__ double row(MF m, ZF z) const {
+    if (LPU == 2) return row_dot4_pair<N, kChunk>(m, z, h);
+    return row_dot4<N, kChunk>(m, z);
+  }
   __device__ __forceinline__ double mv() const {
     const double2* z = vec;
-    const int i = tid;
+    const int i = u;
     if (kLdsM) {
       const double* m = sM;
-      return row_dot4<N, kChunk>([&](int j) { return m[i + j * kLd]; },
-                                 [&](int j) { return z[j].x; });
+      return row([&](int j) { return m[i + j * kLd]; }, [&](int j) { return z[j].x; });
     }
     const double* m = gM;
-    return row_dot4<N, kChunk>([&](int j) { return m[i + (size_t)j * N]; },
-                               [&](int j) { return z[j].x; });
+    return row([&](int j) { return m[i + (size_t)j * N]; }, [&](int j) { return z[j].x; });
   }
   __device__ __forceinline__ double mtv() const {
     const double2* z = vec;
-    const int i = tid;
+    const int i = u;
     if (kLdsM) {
       const double* m = sM;
-      return row_dot4<N, kChunk>([&](int j) { return m[j + i * kLd]; },
-                                 [&](int j) { return z[j].y; });
+      return row([&](int j) { return m[j + i * kLd]; }, [&](int j) { return z[j].y; });
     }
     const double* mt = gMT;
-    return row_dot4<N, kChunk>([&](int j) { return mt[i + (size_t)j * N]; },
-                               [&](int j) { return z[j].y; });
+    return row([&](int j) { return mt[i + (size_t)j * N]; }, [&](int j) { return z[j].y; });
   }
 
   template <class Feat>
