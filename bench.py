@@ -54,11 +54,18 @@ def cpu_baseline(sample=2048, seed=7):
     t0 = time.perf_counter()
     p.solve(mode=pyoracle.LITERAL, nthreads=threads)
     dt = time.perf_counter() - t0
+    # One thread on the first 128 systems of the same sample (a few seconds).
+    one = pyoracle.OracleProblem.from_ensemble(ens.subset(sids[:128]))
+    t1 = time.perf_counter()
+    one.solve(mode=pyoracle.LITERAL, nthreads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": sample / dt, "unit": "system-periods/s", "cores": threads,
             "kind": "port",
             "sample": f"{sample} seeded-random systems (seed {seed}) of the same 16384-system "
                       f"workload, first period from rest, literal host restatement, {threads} "
-                      f"OpenMP threads, {dt:.1f} s"}
+                      f"OpenMP threads, {dt:.1f} s",
+            "single_thread_value": 128 / dt1,
+            "single_thread_sample": f"first 128 systems of that sample, 1 thread, {dt1:.1f} s"}
 
 
 def measured_traffic(kernel_tag):

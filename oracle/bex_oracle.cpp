@@ -11,6 +11,11 @@
 // The canonical mode shares only the deterministic elementary functions (bex_detmath.h)
 // with the product; those are pinned separately against mpmath. The literal mode shares
 // nothing with the product.
+//
+// Additions without a reference counterpart, restated for the kernels' parity tests: one
+// coupling matrix per system is handled by the tests calling the oracle system by system; the
+// interpolated dense output (DenseOutputInterpolation = 1) is pinned against the integrator's
+// own state at the output instants (tests/test_oracle_pins.py).
 #include "bex_oracle.h"
 
 #include <cmath>
