@@ -36,7 +36,8 @@ COMMON_FLAGS = [
 
 def build_library(verbose=False, extra_flags=(), output=LIB):
     sources = [os.path.join(CSRC, "bex_capi.hip"), os.path.join(CSRC, "bex_mlp.hip")]
-    cmd = [HIPCC, *COMMON_FLAGS, *extra_flags, "-shared", "-o", output + ".tmp", *sources]
+    cmd = [HIPCC, *COMMON_FLAGS, *extra_flags, "-shared", "-Wl,--no-undefined", "-o",
+           output + ".tmp", *sources]
     if verbose:
         cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
         print(" ".join(cmd), flush=True)

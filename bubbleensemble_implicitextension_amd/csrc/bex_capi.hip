@@ -49,6 +49,8 @@ struct bex_solver {
   double* d_CouplingFull = nullptr;      // full [c][N][N] expanded from the compact storage
   double* d_CouplingFullT = nullptr;     // transpose of coupling 0
   bool PerSystemCoupling = false;        // pool classes: one full matrix per system
+  struct bex_pool* pool = nullptr;       // pool classes: the owning pool and class index
+  int pool_class = -1;
   double* d_CouplingSys = nullptr;       // [NS][N][N] column-major (pool classes)
   double* d_CouplingSysT = nullptr;      // their transposes (pool classes that stream M)
   double* h_tol = nullptr;               // [2*UD + NE] rtol, atol, event tolerance
@@ -861,9 +863,12 @@ int bex_sync_d2h(bex_solver* s, int var) {
   }
 }
 
+static int pool_class_solve(bex_solver* s);  // pool section below
+
 int bex_solve(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
+  if (s->pool) return pool_class_solve(s);  // per-system matrices: the pool's class kernel
   KernelChoice kc;
   const int mode = s->WarmStart ? bexk::kLearned : (s->d_samples ? bexk::kSampling : bexk::kPlain);
   if (mode == bexk::kLearned && !s->d_net)
@@ -1222,6 +1227,19 @@ int capture(bex_pool* p) {
 
 }  // namespace
 
+// Solve() of a pool class's own solver: that class alone, with the pool's current variant and
+// order, on the solver's stream.
+static int pool_class_solve(bex_solver* s) {
+  bex_pool* p = s->pool;
+  if (s->WarmStart || s->d_samples)
+    return fail(s, BEX_E_UNSUPPORTED,
+                "Solve: pool classes run the plain method (no warm start, no sample collection)");
+  if (s->d.NDO > 0) return fail(s, BEX_E_UNSUPPORTED, "Solve: pool classes have no dense output");
+  const int rc = launch_class(p, s->pool_class, s->stream);
+  if (rc != BEX_OK) return fail(s, rc, p->err);
+  return BEX_OK;
+}
+
 extern "C" {
 
 int bex_pool_variant_count(int units) { return (int)pool_variants(units).size(); }
@@ -1272,6 +1290,8 @@ int bex_pool_create(int nclass, const int* units, const int* systems, int device
     const int rc = bex_create(&d, device, &kk.s);
     if (rc != BEX_OK) return bail(rc, g_create_error);
     kk.s->PerSystemCoupling = true;
+    kk.s->pool = p;
+    kk.s->pool_class = c;
     std::vector<int> order(NS);
     for (int i = 0; i < NS; i++) order[i] = i;
     if (hipMalloc((void**)&kk.s->d_CouplingSys, sizeof(double) * (size_t)NS * N * N) != hipSuccess ||
@@ -1443,57 +1463,77 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
     bex_solver* s = k.s;
     Buf* keep[] = {&s->ActualState, &s->ActualTime, &s->UnitAccessories, &s->IntegerUnitAccessories,
                    &s->SystemAccessories, &s->IntegerSystemAccessories, &s->TimeDomain};
-    std::vector<void*> saved;
-    for (Buf* b : keep) {
-      void* d = nullptr;
-      BEX_PHIP(p, hipMalloc(&d, b->bytes()));
-      BEX_PHIP(p, hipMemcpy(d, b->d, b->bytes(), hipMemcpyDeviceToDevice));
-      saved.push_back(d);
-    }
+    const size_t nkeep = sizeof(keep) / sizeof(keep[0]);
+    std::vector<void*> saved(nkeep, nullptr);
     long long* stats_saved = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     const size_t sbytes = sizeof(long long) * (size_t)k.NS * BEX_NSTAT;
-    BEX_PHIP(p, hipMalloc((void**)&stats_saved, sbytes));
-    BEX_PHIP(p, hipMemcpy(stats_saved, s->d_stats, sbytes, hipMemcpyDeviceToDevice));
+    const int old = k.variant;
+    int rc = BEX_OK;
+    std::string msg;
+    auto check = [&](hipError_t e, const char* what) {
+      if (e != hipSuccess && rc == BEX_OK) {
+        rc = BEX_E_HIP;
+        msg = std::string(what) + ": " + hipGetErrorString(e);
+      }
+      return rc == BEX_OK;
+    };
+    // Snapshot of the class's device state and counters.
+    for (size_t b = 0; b < nkeep && rc == BEX_OK; b++)
+      if (check(hipMalloc(&saved[b], keep[b]->bytes()), "hipMalloc"))
+        check(hipMemcpy(saved[b], keep[b]->d, keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
+    if (check(hipMalloc((void**)&stats_saved, sbytes), "hipMalloc"))
+      check(hipMemcpy(stats_saved, s->d_stats, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
     // Probe time domain: [TD0, TD0 + f (TD1 - TD0)] from the device copy.
     std::vector<double> td(2 * (size_t)k.NS);
-    BEX_PHIP(p, hipMemcpy(td.data(), s->TimeDomain.d, sizeof(double) * td.size(), hipMemcpyDeviceToHost));
+    check(hipMemcpy(td.data(), s->TimeDomain.d, sizeof(double) * td.size(), hipMemcpyDeviceToHost),
+          "hipMemcpy");
     for (int i = 0; i < k.NS; i++) td[k.NS + i] = td[i] + probe_fraction * (td[k.NS + i] - td[i]);
-    hipEvent_t t0, t1;
-    BEX_PHIP(p, hipEventCreate(&t0));
-    BEX_PHIP(p, hipEventCreate(&t1));
+    check(hipEventCreate(&t0), "hipEventCreate");
+    check(hipEventCreate(&t1), "hipEventCreate");
     const int nv = (int)pool_variants(k.N).size();
-    int best = 0;
+    int best = old;
     float best_ms = 0.0f;
-    for (int v = 0; v < nv; v++) {
-      for (size_t b = 0; b < saved.size(); b++)
-        BEX_PHIP(p, hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice));
-      BEX_PHIP(p, hipMemcpy(s->TimeDomain.d, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
-      const int old = k.variant;
+    for (int v = 0; v < nv && rc == BEX_OK; v++) {
+      for (size_t b = 0; b < nkeep; b++)
+        check(hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
+      check(hipMemcpy(s->TimeDomain.d, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice),
+            "hipMemcpy");
+      if (rc != BEX_OK) break;
       k.variant = v;
-      BEX_PHIP(p, hipEventRecord(t0, p->stream));
-      const int rc = launch_class(p, (int)c, p->stream);
-      BEX_PHIP(p, hipEventRecord(t1, p->stream));
-      BEX_PHIP(p, hipEventSynchronize(t1));
-      k.variant = old;
-      if (rc != BEX_OK) return rc;
+      check(hipEventRecord(t0, p->stream), "hipEventRecord");
+      const int lrc = launch_class(p, (int)c, p->stream);
+      check(hipEventRecord(t1, p->stream), "hipEventRecord");
+      check(hipEventSynchronize(t1), "hipEventSynchronize");
+      if (lrc != BEX_OK && rc == BEX_OK) {
+        rc = lrc;
+        msg = p->err;
+      }
       float ms = 0.0f;
-      BEX_PHIP(p, hipEventElapsedTime(&ms, t0, t1));
-      if (times_ms) times_ms[slot + v] = ms;
-      if (v == 0 || ms < best_ms) {
-        best_ms = ms;
-        best = v;
+      if (check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime")) {
+        if (times_ms) times_ms[slot + v] = ms;
+        if (v == 0 || ms < best_ms) {
+          best_ms = ms;
+          best = v;
+        }
       }
     }
     slot += nv;
-    for (size_t b = 0; b < saved.size(); b++) {
-      BEX_PHIP(p, hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice));
-      hipFree(saved[b]);
+    // Restore the snapshot whatever happened, then release it.
+    for (size_t b = 0; b < nkeep; b++)
+      if (saved[b]) check(hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice),
+                          "hipMemcpy");
+    if (stats_saved) check(hipMemcpy(s->d_stats, stats_saved, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+    for (void* d : saved)
+      if (d) hipFree(d);
+    if (stats_saved) hipFree(stats_saved);
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
+    k.variant = (rc == BEX_OK) ? best : old;
+    if (rc != BEX_OK) {
+      p->graph_valid = false;
+      return pfail(p, rc, "bex_pool_autotune: " + msg);
     }
-    BEX_PHIP(p, hipMemcpy(s->d_stats, stats_saved, sbytes, hipMemcpyDeviceToDevice));
-    hipFree(stats_saved);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    k.variant = best;
   }
   p->graph_valid = false;
   return BEX_OK;

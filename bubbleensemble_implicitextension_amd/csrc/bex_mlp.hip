@@ -12,6 +12,7 @@
 #include "bex_mlp.h"
 
 namespace bexm {
+namespace {  // internal linkage: only the extern "C" entry points are exported
 
 constexpr int H = BEX_MLP_HIDDEN;
 constexpr int TS = 64;     // samples per block (one wave)
@@ -177,6 +178,7 @@ int allow_lds() {
   return BEX_OK;
 }
 
+}  // namespace
 }  // namespace bexm
 
 extern "C" {

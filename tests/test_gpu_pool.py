@@ -153,3 +153,16 @@ def test_config4_class_matches_oracle(oracle):
     res = _results(pool, classes)
     pool.close()
     _check_against_oracle(oracle, classes, res, 1, 0.02)
+
+
+def test_class_solver_solve_uses_its_own_matrices(oracle):
+    """Solve() on a pool class's own solver runs that class alone with each system's matrix
+    (not the first system's): every system matches the oracle."""
+    classes = [wl.config2_class(64, 3, seed=14), wl.config2_class(16, 5, seed=14)]
+    print("case: class solver Solve(), N=64 and N=16", flush=True)
+    pool = _pool(classes)
+    for c in range(len(classes)):
+        pool.solver(c).Solve()
+    res = _results(pool, classes)
+    pool.close()
+    _check_against_oracle(oracle, classes, res, 1, END)
