@@ -29,10 +29,15 @@ namespace bex {
 // scalar register pair right where it is used: without the opaque move, loop-invariant code
 // motion lifts every coefficient of the inlined functions out of the solver's time loop into
 // vector registers, and the register allocator then spills them (and the coupling rows) to
-// scratch memory on every stage. The value itself is unchanged.
-BEX_HD double kc(double c) {
+// scratch memory on every stage. The empty asm takes the function's varying argument `dep` as
+// an input, so it cannot be hoisted, yet it is not volatile, so the scheduler may interleave
+// independent evaluations (pow, sincospi, sincos of one right-hand side). The value itself is
+// unchanged.
+BEX_HD double kc(double c, double dep) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  asm volatile("" : "+s"(c));
+  asm("" : "+s"(c) : "v"(dep));
+#else
+  (void)dep;
 #endif
   return c;
 }
@@ -57,60 +62,60 @@ BEX_HD double fabs_d(double x) { return from_bits(as_bits(x) & 0x7ffffffffffffff
 
 // sin(pi*r) - pi*r, for |r| <= 1/4, as r^3 * P(r^2).
 BEX_HD double sinpi_tail(double r, double r2) {
-  double p = kc(0x1.2877020d52cf0p-31);
-  p = fma(p, r2, kc(-0x1.8a404211f9547p-26));
-  p = fma(p, r2, kc(0x1.aaec32af93359p-21));
-  p = fma(p, r2, kc(-0x1.6fadb9f155744p-16));
-  p = fma(p, r2, kc(0x1.e8f434d018d63p-12));
-  p = fma(p, r2, kc(-0x1.e3074fde8871fp-8));
-  p = fma(p, r2, kc(0x1.50783487ee782p-4));
-  p = fma(p, r2, kc(-0x1.32d2cce62bd86p-1));
-  p = fma(p, r2, kc(0x1.466bc6775aae2p+1));
-  p = fma(p, r2, kc(-0x1.4abbce625be53p+2));
+  double p = kc(0x1.2877020d52cf0p-31, r2);
+  p = fma(p, r2, kc(-0x1.8a404211f9547p-26, r2));
+  p = fma(p, r2, kc(0x1.aaec32af93359p-21, r2));
+  p = fma(p, r2, kc(-0x1.6fadb9f155744p-16, r2));
+  p = fma(p, r2, kc(0x1.e8f434d018d63p-12, r2));
+  p = fma(p, r2, kc(-0x1.e3074fde8871fp-8, r2));
+  p = fma(p, r2, kc(0x1.50783487ee782p-4, r2));
+  p = fma(p, r2, kc(-0x1.32d2cce62bd86p-1, r2));
+  p = fma(p, r2, kc(0x1.466bc6775aae2p+1, r2));
+  p = fma(p, r2, kc(-0x1.4abbce625be53p+2, r2));
   return (r * r2) * p;
 }
 
 // cos(pi*r) - 1, for |r| <= 1/4, as r^2 * Q(r^2).
 BEX_HD double cospi_tail(double r2) {
-  double q = kc(0x1.ef6e308d6d1c4p-29);
-  q = fma(q, r2, kc(-0x1.2a0c591af8314p-23));
-  q = fma(q, r2, kc(0x1.20c62c2f2d7f5p-18));
-  q = fma(q, r2, kc(-0x1.b6e24f44b128fp-14));
-  q = fma(q, r2, kc(0x1.f9d38a3763cc3p-10));
-  q = fma(q, r2, kc(-0x1.a6d1f2a204a8cp-6));
-  q = fma(q, r2, kc(0x1.e1f506891babbp-3));
-  q = fma(q, r2, kc(-0x1.55d3c7e3cbffap+0));
-  q = fma(q, r2, kc(0x1.03c1f081b5ac4p+2));
-  q = fma(q, r2, kc(-0x1.3bd3cc9be45dep+2));
+  double q = kc(0x1.ef6e308d6d1c4p-29, r2);
+  q = fma(q, r2, kc(-0x1.2a0c591af8314p-23, r2));
+  q = fma(q, r2, kc(0x1.20c62c2f2d7f5p-18, r2));
+  q = fma(q, r2, kc(-0x1.b6e24f44b128fp-14, r2));
+  q = fma(q, r2, kc(0x1.f9d38a3763cc3p-10, r2));
+  q = fma(q, r2, kc(-0x1.a6d1f2a204a8cp-6, r2));
+  q = fma(q, r2, kc(0x1.e1f506891babbp-3, r2));
+  q = fma(q, r2, kc(-0x1.55d3c7e3cbffap+0, r2));
+  q = fma(q, r2, kc(0x1.03c1f081b5ac4p+2, r2));
+  q = fma(q, r2, kc(-0x1.3bd3cc9be45dep+2, r2));
   return r2 * q;
 }
 
 // sin(r) - r and cos(r) - 1 for |r| <= pi/4 (+ reduction slack).
 BEX_HD double sin_tail(double r, double r2) {
-  double p = kc(0x1.71b8ef6dcf572p-66);
-  p = fma(p, r2, kc(-0x1.2f49b46814157p-57));
-  p = fma(p, r2, kc(0x1.952c77030ad4ap-49));
-  p = fma(p, r2, kc(-0x1.ae7f3e733b81fp-41));
-  p = fma(p, r2, kc(0x1.6124613a86d09p-33));
-  p = fma(p, r2, kc(-0x1.ae64567f544e4p-26));
-  p = fma(p, r2, kc(0x1.71de3a556c734p-19));
-  p = fma(p, r2, kc(-0x1.a01a01a01a01ap-13));
-  p = fma(p, r2, kc(0x1.1111111111111p-7));
-  p = fma(p, r2, kc(-0x1.5555555555555p-3));
+  double p = kc(0x1.71b8ef6dcf572p-66, r2);
+  p = fma(p, r2, kc(-0x1.2f49b46814157p-57, r2));
+  p = fma(p, r2, kc(0x1.952c77030ad4ap-49, r2));
+  p = fma(p, r2, kc(-0x1.ae7f3e733b81fp-41, r2));
+  p = fma(p, r2, kc(0x1.6124613a86d09p-33, r2));
+  p = fma(p, r2, kc(-0x1.ae64567f544e4p-26, r2));
+  p = fma(p, r2, kc(0x1.71de3a556c734p-19, r2));
+  p = fma(p, r2, kc(-0x1.a01a01a01a01ap-13, r2));
+  p = fma(p, r2, kc(0x1.1111111111111p-7, r2));
+  p = fma(p, r2, kc(-0x1.5555555555555p-3, r2));
   return (r * r2) * p;
 }
 BEX_HD double cos_tail(double r2) {
-  double q = kc(-0x1.0ce396db7f853p-70);
-  q = fma(q, r2, kc(0x1.e542ba4020225p-62));
-  q = fma(q, r2, kc(-0x1.6827863b97d97p-53));
-  q = fma(q, r2, kc(0x1.ae7f3e733b81fp-45));
-  q = fma(q, r2, kc(-0x1.93974a8c07c9dp-37));
-  q = fma(q, r2, kc(0x1.1eed8eff8d898p-29));
-  q = fma(q, r2, kc(-0x1.27e4fb7789f5cp-22));
-  q = fma(q, r2, kc(0x1.a01a01a01a01ap-16));
-  q = fma(q, r2, kc(-0x1.6c16c16c16c17p-10));
-  q = fma(q, r2, kc(0x1.5555555555555p-5));
-  q = fma(q, r2, kc(-0x1.0000000000000p-1));
+  double q = kc(-0x1.0ce396db7f853p-70, r2);
+  q = fma(q, r2, kc(0x1.e542ba4020225p-62, r2));
+  q = fma(q, r2, kc(-0x1.6827863b97d97p-53, r2));
+  q = fma(q, r2, kc(0x1.ae7f3e733b81fp-45, r2));
+  q = fma(q, r2, kc(-0x1.93974a8c07c9dp-37, r2));
+  q = fma(q, r2, kc(0x1.1eed8eff8d898p-29, r2));
+  q = fma(q, r2, kc(-0x1.27e4fb7789f5cp-22, r2));
+  q = fma(q, r2, kc(0x1.a01a01a01a01ap-16, r2));
+  q = fma(q, r2, kc(-0x1.6c16c16c16c17p-10, r2));
+  q = fma(q, r2, kc(0x1.5555555555555p-5, r2));
+  q = fma(q, r2, kc(-0x1.0000000000000p-1, r2));
   return r2 * q;
 }
 
@@ -137,7 +142,7 @@ BEX_HD void sincospi(double x, double* s, double* c) {
   int q = (int)q4;
   double r2 = r * r;
   // sin(pi r) = r*pi_hi + (r*pi_lo + tail)
-  double sp = fma(r, kc(0x1.921fb54442d18p+1), fma(r, kc(0x1.1a62633145c07p-53), sinpi_tail(r, r2)));
+  double sp = fma(r, kc(0x1.921fb54442d18p+1, r), fma(r, kc(0x1.1a62633145c07p-53, r), sinpi_tail(r, r2)));
   double cp = 1.0 + cospi_tail(r2);
   switch (q) {
     case 0: *s = sp; *c = cp; break;
@@ -151,10 +156,10 @@ BEX_HD void sincospi(double x, double* s, double* c) {
 // |x| < 2^20 * pi/2 (~1.6e6); beyond that the result stays finite but loses accuracy. The
 // solver evaluates these at 2*pi*f2/f1*tau + theta with tau in [0, 1].
 BEX_HD void reduce_pio2(double x, double* r, int* q) {
-  double n = rint(x * kc(0x1.45f306dc9c883p-1));
-  double t = fma(-n, kc(0x1.921fb54400000p+0), x);
-  t = fma(-n, kc(0x1.0b4611a600000p-34), t);
-  t = fma(-n, kc(0x1.3198a2e000000p-69), t);
+  double n = rint(x * kc(0x1.45f306dc9c883p-1, x));
+  double t = fma(-n, kc(0x1.921fb54400000p+0, x), x);
+  t = fma(-n, kc(0x1.0b4611a600000p-34, x), t);
+  t = fma(-n, kc(0x1.3198a2e000000p-69, x), t);
   *r = t;
   double q4 = n - 4.0 * floor(n * 0.25);
   *q = (int)q4;
@@ -238,30 +243,30 @@ BEX_HD double log_dd(double x, double* lo) {
   double e = fma(-sh, dh, f) - sh * dl;       // f - sh*(dh+dl)
   double sl = e * rinv;
   double s2 = sh * sh;
-  double p = kc(0x1.2f684bda12f68p-5);            // 1/27
-  p = fma(p, s2, kc(0x1.47ae147ae147bp-5));       // 1/25
-  p = fma(p, s2, kc(0x1.642c8590b2164p-5));       // 1/23
-  p = fma(p, s2, kc(0x1.8618618618618p-5));       // 1/21
-  p = fma(p, s2, kc(0x1.af286bca1af28p-5));       // 1/19
-  p = fma(p, s2, kc(0x1.e1e1e1e1e1e1ep-5));       // 1/17
-  p = fma(p, s2, kc(0x1.1111111111111p-4));       // 1/15
-  p = fma(p, s2, kc(0x1.3b13b13b13b14p-4));       // 1/13
-  p = fma(p, s2, kc(0x1.745d1745d1746p-4));       // 1/11
-  p = fma(p, s2, kc(0x1.c71c71c71c71cp-4));       // 1/9
-  p = fma(p, s2, kc(0x1.2492492492492p-3));       // 1/7
-  p = fma(p, s2, kc(0x1.999999999999ap-3));       // 1/5
-  p = fma(p, s2, kc(0x1.5555555555555p-2));       // 1/3
+  double p = kc(0x1.2f684bda12f68p-5, s2);            // 1/27
+  p = fma(p, s2, kc(0x1.47ae147ae147bp-5, s2));       // 1/25
+  p = fma(p, s2, kc(0x1.642c8590b2164p-5, s2));       // 1/23
+  p = fma(p, s2, kc(0x1.8618618618618p-5, s2));       // 1/21
+  p = fma(p, s2, kc(0x1.af286bca1af28p-5, s2));       // 1/19
+  p = fma(p, s2, kc(0x1.e1e1e1e1e1e1ep-5, s2));       // 1/17
+  p = fma(p, s2, kc(0x1.1111111111111p-4, s2));       // 1/15
+  p = fma(p, s2, kc(0x1.3b13b13b13b14p-4, s2));       // 1/13
+  p = fma(p, s2, kc(0x1.745d1745d1746p-4, s2));       // 1/11
+  p = fma(p, s2, kc(0x1.c71c71c71c71cp-4, s2));       // 1/9
+  p = fma(p, s2, kc(0x1.2492492492492p-3, s2));       // 1/7
+  p = fma(p, s2, kc(0x1.999999999999ap-3, s2));       // 1/5
+  p = fma(p, s2, kc(0x1.5555555555555p-2, s2));       // 1/3
   double tail = (sh * s2) * p;                // s^3 (1/3 + s^2/5 + ...)
   double lm_hi = 2.0 * sh;
   double lm_lo = 2.0 * (sl + tail);
   // + k*ln2, ln2 split so that k*LN2_A is exact.
   double kd = (double)k;
-  double a = kd * kc(0x1.62e42ff000000p-1);
+  double a = kd * kc(0x1.62e42ff000000p-1, kd);
   double bsum = a + lm_hi;
   // two-sum of a and lm_hi
   double bb = bsum - a;
   double err = (a - (bsum - bb)) + (lm_hi - bb);
-  double low = err + (lm_lo + kd * kc(-0x1.718432a1b0e26p-35));
+  double low = err + (lm_lo + kd * kc(-0x1.718432a1b0e26p-35, kd));
   double hi = bsum + low;
   *lo = low - (hi - bsum);
   return hi;
@@ -271,22 +276,22 @@ BEX_HD double log_dd(double x, double* lo) {
 BEX_HD double exp_dd(double th, double tl) {
   if (th > 709.782712893384) return infinity();
   if (th < -745.1332191019412) return 0.0;
-  double n = rint(th * kc(0x1.71547652b82fep+0));
-  double r = fma(-n, kc(0x1.62e42ff000000p-1), th);   // exact (Sterbenz region)
-  r = r + (tl - n * kc(-0x1.718432a1b0e26p-35));
-  double p = kc(0x1.93974a8c07c9dp-37);               // 1/14!
-  p = fma(p, r, kc(0x1.6124613a86d09p-33));
-  p = fma(p, r, kc(0x1.1eed8eff8d898p-29));
-  p = fma(p, r, kc(0x1.ae64567f544e4p-26));
-  p = fma(p, r, kc(0x1.27e4fb7789f5cp-22));
-  p = fma(p, r, kc(0x1.71de3a556c734p-19));
-  p = fma(p, r, kc(0x1.a01a01a01a01ap-16));
-  p = fma(p, r, kc(0x1.a01a01a01a01ap-13));
-  p = fma(p, r, kc(0x1.6c16c16c16c17p-10));
-  p = fma(p, r, kc(0x1.1111111111111p-7));
-  p = fma(p, r, kc(0x1.5555555555555p-5));
-  p = fma(p, r, kc(0x1.5555555555555p-3));
-  p = fma(p, r, kc(0x1.0000000000000p-1));
+  double n = rint(th * kc(0x1.71547652b82fep+0, th));
+  double r = fma(-n, kc(0x1.62e42ff000000p-1, th), th);   // exact (Sterbenz region)
+  r = r + (tl - n * kc(-0x1.718432a1b0e26p-35, th));
+  double p = kc(0x1.93974a8c07c9dp-37, r);               // 1/14!
+  p = fma(p, r, kc(0x1.6124613a86d09p-33, r));
+  p = fma(p, r, kc(0x1.1eed8eff8d898p-29, r));
+  p = fma(p, r, kc(0x1.ae64567f544e4p-26, r));
+  p = fma(p, r, kc(0x1.27e4fb7789f5cp-22, r));
+  p = fma(p, r, kc(0x1.71de3a556c734p-19, r));
+  p = fma(p, r, kc(0x1.a01a01a01a01ap-16, r));
+  p = fma(p, r, kc(0x1.a01a01a01a01ap-13, r));
+  p = fma(p, r, kc(0x1.6c16c16c16c17p-10, r));
+  p = fma(p, r, kc(0x1.1111111111111p-7, r));
+  p = fma(p, r, kc(0x1.5555555555555p-5, r));
+  p = fma(p, r, kc(0x1.5555555555555p-3, r));
+  p = fma(p, r, kc(0x1.0000000000000p-1, r));
   double em1 = fma(r * r, p, r);
   double res = 1.0 + em1;
   int ni = (int)n;
