@@ -127,59 +127,87 @@ __device__ __forceinline__ void fmac_row16(double* acc, double z, const double* 
         "v"(m[14]), "v"(m[15]));
 }
 
+// A second register copy of v. A permlane swap overwrites both of its operands; handing it v
+// and this copy costs one 64-bit move.
+__device__ __forceinline__ double twin(double v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
 // Row exchange inside each 32-lane half: returns the value of the half's even row and of its
 // odd row at the lane's position in the row, i.e. z[c] and z[16 + c] of the lane's half.
 __device__ __forceinline__ void rows_of_half(double z, double* even, double* odd) {
-  const unsigned lo = __double2loint(z), hi = __double2hiint(z);
-  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const double c = twin(z);
+  const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(z), __double2loint(c), false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(z), __double2hiint(c), false, false);
   *even = __hiloint2double((int)h[0], (int)l[0]);
   *odd = __hiloint2double((int)h[1], (int)l[1]);
 }
 
 // Value of the lower half and of the upper half at lanes i and i + 32.
 __device__ __forceinline__ void halves(double z, double* lower, double* upper) {
-  const unsigned lo = __double2loint(z), hi = __double2hiint(z);
-  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const double c = twin(z);
+  const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(z), __double2loint(c), false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(z), __double2hiint(c), false, false);
   *lower = __hiloint2double((int)h[0], (int)l[0]);
   *upper = __hiloint2double((int)h[1], (int)l[1]);
 }
 
-// Halving tree over lanes 0..31 (Steppers.cuh:733-742): s[i] = op(s[i], s[i + h]) for
-// h = 16, 8, 4, 2, 1; the result is lane 0's, broadcast to the wave.
+// DPP row_ror:k (k = 8, 4, 2, 1): the value of lane (c - k) mod 16 of the lane's 16-lane row.
+template <int K>
+__device__ __forceinline__ double row_ror(double v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x120 + K, 0xf, 0xf, true);
+}
+
+// Halving tree over each 32-lane half (Steppers.cuh:733-742): s[i] = op(s[i], s[i + h]) for
+// h = 16, 8, 4, 2, 1, evaluated as a butterfly so that every lane ends with its half's result
+// (no broadcast). At each level a lane combines its value with that of a lane whose index
+// differs by h modulo 2h; after the level both lanes of such a pair hold the same value, so the
+// lane that computed op(upper, lower) holds op(lower, upper), bit for bit, when op is
+// commutative (IEEE addition; VmaxOp). Lane 0 (lane 32) computes exactly the reference's tree.
 template <class Op>
 __device__ __forceinline__ double tree32(double s, Op op) {
   double even, odd;
   rows_of_half(s, &even, &odd);
-  s = op(s, odd);
-  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x108, 0xf, 0xf, true));  // row_shl:8
-  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x104, 0xf, 0xf, true));  // row_shl:4
-  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x102, 0xf, 0xf, true));  // row_shl:2
-  s = op(s, __builtin_amdgcn_mov_dpp(s, 0x101, 0xf, 0xf, true));  // row_shl:1
-  return uniform(s);
+  s = op(even, odd);
+  s = op(s, row_ror<8>(s));
+  s = op(s, row_ror<4>(s));
+  s = op(s, row_ror<2>(s));
+  s = op(s, row_ror<1>(s));
+  return s;
 }
 
-// Two independent halving trees evaluated in lockstep, so that each one's cross-lane moves
-// fill the other's dependency stalls; same per-tree order as tree32.
+// Two independent trees (as tree32) evaluated in lockstep, so that each one's cross-lane moves
+// fill the other's dependency stalls.
 template <class OpA, class OpB>
 __device__ __forceinline__ void tree32x2(double& a, double& b, OpA opa, OpB opb) {
   double ea, oa, eb, ob;
   rows_of_half(a, &ea, &oa);
   rows_of_half(b, &eb, &ob);
-  a = opa(a, oa);
-  b = opb(b, ob);
-  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x108, 0xf, 0xf, true));
-  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x108, 0xf, 0xf, true));
-  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x104, 0xf, 0xf, true));
-  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x104, 0xf, 0xf, true));
-  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x102, 0xf, 0xf, true));
-  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x102, 0xf, 0xf, true));
-  a = opa(a, __builtin_amdgcn_mov_dpp(a, 0x101, 0xf, 0xf, true));
-  b = opb(b, __builtin_amdgcn_mov_dpp(b, 0x101, 0xf, 0xf, true));
-  a = uniform(a);
-  b = uniform(b);
+  a = opa(ea, oa);
+  b = opb(eb, ob);
+  a = opa(a, row_ror<8>(a));
+  b = opb(b, row_ror<8>(b));
+  a = opa(a, row_ror<4>(a));
+  b = opb(b, row_ror<4>(b));
+  a = opa(a, row_ror<2>(a));
+  b = opb(b, row_ror<2>(b));
+  a = opa(a, row_ror<1>(a));
+  b = opb(b, row_ror<1>(b));
 }
+
+// max(a, b) for operands that are >= +0 or a quiet NaN, in one v_max_f64 (IEEE maxNum: a NaN
+// operand yields the other one). On such operands it equals bex::fmax_d (the reference's fmax,
+// MathFunction.cuh:9-38) and is commutative. Written as asm because the compiler's maxnum
+// first re-quiets both operands (two more instructions) for signalling NaNs, which arithmetic
+// never produces.
+struct VmaxOp {
+  __device__ __forceinline__ double operator()(double a, double b) const {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+};
 
 struct AddOp {
   __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
@@ -805,15 +833,14 @@ struct RolePairLayout32 {
     return (acc[0] + acc[1]) + (acc[2] + acc[3]);
   }
 
-  // sum_i a_i b_i (tree order) for a on the primal half and b on the shadow half of v.
+  // sum_i a_i b_i (tree order) for a on the primal half and b on the shadow half of v, in every
+  // lane.
   __device__ __forceinline__ double dot_halves(double v) const {
     double lo, up;
     halves(v, &lo, &up);
     return tree32(lo * up, AddOp());
   }
 
-  // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
-  // meaningful on the primal half.
   // Initial residual r0 = b - A x0 of the primal half, on both halves.
   __device__ __forceinline__ double residual(double b, double x0, double x1sq, double rD) const {
     const double ax = fma(rD, matvec(x1sq * x0), x0);
@@ -822,42 +849,49 @@ struct RolePairLayout32 {
     return lo;
   }
 
+  // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
+  // meaningful on the primal half. The loop is rotated: the reference's iteration-1 branch (the
+  // rho exit, p = r) runs before it and the direction update p = r + beta p of iteration k + 1
+  // closes iteration k, which leaves the arithmetic and its order unchanged. Scalars (rho,
+  // alpha, beta, max |r|) are identical in every lane, so the branches on them are wave-uniform.
   __device__ __forceinline__ BicgOut bicg(double res0, double x0, double x1sq, double rD,
                                           const BexKernelArgs& a, double* xout) const {
     double res = res0;  // r on the primal half, r~ = r on the shadow half
     const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
     const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
-    double dir = 0.0, x = x0, rho2 = 0.0;
-    double rho1 = dot_halves(res);  // r . r~
+    double x = x0;
     BicgOut out{0, false, true, bex::infinity()};
-    for (int iter = 1; iter <= a.BiCGMaxIterations; iter++) {
-      if (iter == 1) {
-        dir = res;
-        if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
-          out.rho_exit = true;
-          out.maxit = false;
-          break;
-        }
-      } else {
-        const double beta = rho1 / rho2;
-        dir = res + beta * dir;  // p (primal), p~ (shadow)
-      }
-      const double qv = fma(s2, matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
-      out.iterations++;
-      const double alpha = rho1 / dot_halves(primal_ ? qv : dir);  // p~ . q
-      x = x + alpha * dir;
-      res = res - alpha * qv;
-      rho2 = rho1;
-      // The residual test and the next iteration's r.r~ are independent: evaluate together.
-      double lo2, up2;
-      halves(res, &lo2, &up2);
-      double rmax = bex::fabs_d(res);
-      rho1 = lo2 * up2;
-      tree32x2(rmax, rho1, FmaxOp(), AddOp());
-      out.rmax = rmax;
-      if (rmax < a.BiCGResidualTolerance) {
+    const int maxit = a.BiCGMaxIterations;
+    if (maxit >= 1) {
+      double rho1 = dot_halves(res);  // r . r~
+      if (any(bex::fabs_d(rho1) < a.BiCGRhoTolerance)) {
+        out.rho_exit = true;
         out.maxit = false;
-        break;
+      } else {
+        double dir = res;  // p (primal), p~ (shadow)
+        for (int iter = 1;; iter++) {
+          const double qv = fma(s2, matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
+          out.iterations++;
+          const double alpha = rho1 / dot_halves(primal_ ? qv : dir);  // p~ . q
+          x = x + alpha * dir;
+          res = res - alpha * qv;
+          const double rho2 = rho1;
+          // The residual test (max |r| over the primal half, evaluated on both halves) and the
+          // next r . r~ are independent: reduce them together.
+          double lo2, up2;
+          halves(res, &lo2, &up2);
+          double rmax = bex::fabs_d(lo2);
+          rho1 = lo2 * up2;
+          tree32x2(rmax, rho1, VmaxOp(), AddOp());
+          out.rmax = rmax;
+          if (any(rmax < a.BiCGResidualTolerance)) {
+            out.maxit = false;
+            break;
+          }
+          if (iter >= maxit) break;
+          const double beta = rho1 / rho2;
+          dir = res + beta * dir;
+        }
       }
     }
     *xout = x;
@@ -897,8 +931,8 @@ struct RolePairLayout32 {
       const double rl = residual(b, xl, x1sq, rD);
       double ml = bex::fabs_d(rl);
       mu = bex::fabs_d(r0);
-      tree32x2(mu, ml, FmaxOp(), FmaxOp());
-      if (ml < mu) {
+      tree32x2(mu, ml, VmaxOp(), VmaxOp());
+      if (any(ml < mu)) {
         learned_start = true;
         x0 = xl;
         r0 = rl;
@@ -908,7 +942,7 @@ struct RolePairLayout32 {
     BicgOut out = bicg(r0, x0, x1sq, rD, a, &x);
     // A learned start that ends unconverged (iteration cap) and no closer than the plain start
     // began, or non-finite, is discarded: plain restart from the uncoupled acceleration.
-    if (MODE == kLearned && learned_start && out.maxit && !(out.rmax < mu)) {
+    if (MODE == kLearned && learned_start && out.maxit && any(!(out.rmax < mu))) {
       const BicgOut again = bicg(residual(b, xu, x1sq, rD), xu, x1sq, rD, a, &x);
       if (lane == 0) stat[BEX_STAT_WARM_FALLBACKS]++;
       out.iterations += again.iterations;
