@@ -25,21 +25,96 @@
 
 namespace bex {
 
-// A polynomial coefficient or splitting constant. On the device the value is pinned into a
-// scalar register pair right where it is used: without the opaque move, loop-invariant code
-// motion lifts every coefficient of the inlined functions out of the solver's time loop into
-// vector registers, and the register allocator then spills them (and the coupling rows) to
-// scratch memory on every stage. The empty asm takes the function's varying argument `dep` as
-// an input, so it cannot be hoisted, yet it is not volatile, so the scheduler may interleave
-// independent evaluations (pow, sincospi, sincos of one right-hand side). The value itself is
-// unchanged.
-BEX_HD double kc(double c, double dep) {
+// Polynomial coefficients and splitting constants, one table (Taylor coefficients rounded to
+// double, generated with mpmath). On the device the table sits in constant memory and is read
+// with scalar loads at the point of use: Coef(dep) passes the table's address through an empty
+// asm that takes the evaluation's varying argument as an input, so loop-invariant code motion
+// cannot lift the loads out of the solver's time loop (held across the loop, ~80 coefficients
+// would be spilled), while neighbouring coefficients still merge into wide scalar loads. A
+// constant written as an immediate would instead cost two scalar moves per use, since 64-bit
+// VOP3 operands take no literal on gfx950.
+enum CoefIndex : int {
+  kSinpiTail = 0,
+  kCospiTail = 10,
+  kPi = 20,
+  kPio2 = 22,
+  kSinTail = 26,
+  kCosTail = 36,
+  kLog = 47,
+  kExp = 62,
+  kNumCoef = 78
+};
+
+#define BEX_DETMATH_COEFFICIENTS \
+    /* kSinpiTail: sin(pi r) - pi r: r^2-Horner coefficients, highest order first */ \
+    0x1.2877020d52cf0p-31, -0x1.8a404211f9547p-26, 0x1.aaec32af93359p-21, \
+    -0x1.6fadb9f155744p-16, 0x1.e8f434d018d63p-12, -0x1.e3074fde8871fp-8, \
+    0x1.50783487ee782p-4, -0x1.32d2cce62bd86p-1, 0x1.466bc6775aae2p+1, \
+    -0x1.4abbce625be53p+2, \
+    /* kCospiTail: cos(pi r) - 1: r^2-Horner coefficients, highest order first */ \
+    0x1.ef6e308d6d1c4p-29, -0x1.2a0c591af8314p-23, 0x1.20c62c2f2d7f5p-18, \
+    -0x1.b6e24f44b128fp-14, 0x1.f9d38a3763cc3p-10, -0x1.a6d1f2a204a8cp-6, \
+    0x1.e1f506891babbp-3, -0x1.55d3c7e3cbffap+0, 0x1.03c1f081b5ac4p+2, \
+    -0x1.3bd3cc9be45dep+2, \
+    /* kPi: pi = hi + lo */ \
+    0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53, \
+    /* kPio2: 2/pi; pi/2 in three parts (Cody-Waite) */ \
+    0x1.45f306dc9c883p-1, 0x1.921fb54400000p+0, 0x1.0b4611a600000p-34, \
+    0x1.3198a2e000000p-69, \
+    /* kSinTail: sin(r) - r: r^2-Horner coefficients, highest order first */ \
+    0x1.71b8ef6dcf572p-66, -0x1.2f49b46814157p-57, 0x1.952c77030ad4ap-49, \
+    -0x1.ae7f3e733b81fp-41, 0x1.6124613a86d09p-33, -0x1.ae64567f544e4p-26, \
+    0x1.71de3a556c734p-19, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7, \
+    -0x1.5555555555555p-3, \
+    /* kCosTail: cos(r) - 1: r^2-Horner coefficients, highest order first */ \
+    -0x1.0ce396db7f853p-70, 0x1.e542ba4020225p-62, -0x1.6827863b97d97p-53, \
+    0x1.ae7f3e733b81fp-45, -0x1.93974a8c07c9dp-37, 0x1.1eed8eff8d898p-29, \
+    -0x1.27e4fb7789f5cp-22, 0x1.a01a01a01a01ap-16, -0x1.6c16c16c16c17p-10, \
+    0x1.5555555555555p-5, -0x1.0000000000000p-1, \
+    /* kLog: log(m) = 2s + 2s^3 P(s^2), P: 1/27 ... 1/3; ln 2 = a + b, k*a exact */ \
+    0x1.2f684bda12f68p-5, 0x1.47ae147ae147bp-5, 0x1.642c8590b2164p-5, \
+    0x1.8618618618618p-5, 0x1.af286bca1af28p-5, 0x1.e1e1e1e1e1e1ep-5, \
+    0x1.1111111111111p-4, 0x1.3b13b13b13b14p-4, 0x1.745d1745d1746p-4, \
+    0x1.c71c71c71c71cp-4, 0x1.2492492492492p-3, 0x1.999999999999ap-3, \
+    0x1.5555555555555p-2, 0x1.62e42ff000000p-1, -0x1.718432a1b0e26p-35, \
+    /* kExp: 1/ln 2; ln 2 = a + b; exp(r) = 1 + r + r^2 P(r), P: 1/14! ... 1/2 */ \
+    0x1.71547652b82fep+0, 0x1.62e42ff000000p-1, -0x1.718432a1b0e26p-35, \
+    0x1.93974a8c07c9dp-37, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29, \
+    0x1.ae64567f544e4p-26, 0x1.27e4fb7789f5cp-22, 0x1.71de3a556c734p-19, \
+    0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-13, 0x1.6c16c16c16c17p-10, \
+    0x1.1111111111111p-7, 0x1.5555555555555p-5, 0x1.5555555555555p-3, \
+    0x1.0000000000000p-1
+
 #if defined(__HIP_DEVICE_COMPILE__)
-  asm("" : "+s"(c) : "v"(dep));
-#else
-  (void)dep;
+static __constant__ const double kCoefDevice[kNumCoef] = {BEX_DETMATH_COEFFICIENTS};
 #endif
-  return c;
+static constexpr double kCoefHost[kNumCoef] = {BEX_DETMATH_COEFFICIENTS};
+
+struct Coef {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) double* p;  // constant memory: scalar loads
+  __device__ explicit Coef(double dep)
+      : p((const __attribute__((address_space(4))) double*)kCoefDevice) {
+    asm("" : "+s"(p) : "v"(dep));
+  }
+#else
+  const double* p;
+  explicit Coef(double) : p(kCoefHost) {}
+#endif
+  BEX_HD double operator[](int i) const { return p[i]; }
+};
+
+// One Horner step p * x + c with a table coefficient c, as a single fma. On the device the
+// coefficient stays in its scalar registers as the addend of one v_fma_f64; left to itself the
+// compiler copies it into vector registers for a v_fmac_f64 (three instructions).
+BEX_HD double horner(double p, double x, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(x), "s"(c));
+  return r;
+#else
+  return fma(p, x, c);
+#endif
 }
 
 BEX_HD uint64_t as_bits(double x) { return __builtin_bit_cast(uint64_t, x); }
@@ -61,61 +136,61 @@ BEX_HD double fabs_d(double x) { return from_bits(as_bits(x) & 0x7ffffffffffffff
 // Polynomial kernels (Taylor coefficients rounded to double, generated with mpmath).
 
 // sin(pi*r) - pi*r, for |r| <= 1/4, as r^3 * P(r^2).
-BEX_HD double sinpi_tail(double r, double r2) {
-  double p = kc(0x1.2877020d52cf0p-31, r2);
-  p = fma(p, r2, kc(-0x1.8a404211f9547p-26, r2));
-  p = fma(p, r2, kc(0x1.aaec32af93359p-21, r2));
-  p = fma(p, r2, kc(-0x1.6fadb9f155744p-16, r2));
-  p = fma(p, r2, kc(0x1.e8f434d018d63p-12, r2));
-  p = fma(p, r2, kc(-0x1.e3074fde8871fp-8, r2));
-  p = fma(p, r2, kc(0x1.50783487ee782p-4, r2));
-  p = fma(p, r2, kc(-0x1.32d2cce62bd86p-1, r2));
-  p = fma(p, r2, kc(0x1.466bc6775aae2p+1, r2));
-  p = fma(p, r2, kc(-0x1.4abbce625be53p+2, r2));
+BEX_HD double sinpi_tail(double r, double r2, const Coef& C) {
+  double p = C[kSinpiTail];
+  p = horner(p, r2, C[kSinpiTail + 1]);
+  p = horner(p, r2, C[kSinpiTail + 2]);
+  p = horner(p, r2, C[kSinpiTail + 3]);
+  p = horner(p, r2, C[kSinpiTail + 4]);
+  p = horner(p, r2, C[kSinpiTail + 5]);
+  p = horner(p, r2, C[kSinpiTail + 6]);
+  p = horner(p, r2, C[kSinpiTail + 7]);
+  p = horner(p, r2, C[kSinpiTail + 8]);
+  p = horner(p, r2, C[kSinpiTail + 9]);
   return (r * r2) * p;
 }
 
 // cos(pi*r) - 1, for |r| <= 1/4, as r^2 * Q(r^2).
-BEX_HD double cospi_tail(double r2) {
-  double q = kc(0x1.ef6e308d6d1c4p-29, r2);
-  q = fma(q, r2, kc(-0x1.2a0c591af8314p-23, r2));
-  q = fma(q, r2, kc(0x1.20c62c2f2d7f5p-18, r2));
-  q = fma(q, r2, kc(-0x1.b6e24f44b128fp-14, r2));
-  q = fma(q, r2, kc(0x1.f9d38a3763cc3p-10, r2));
-  q = fma(q, r2, kc(-0x1.a6d1f2a204a8cp-6, r2));
-  q = fma(q, r2, kc(0x1.e1f506891babbp-3, r2));
-  q = fma(q, r2, kc(-0x1.55d3c7e3cbffap+0, r2));
-  q = fma(q, r2, kc(0x1.03c1f081b5ac4p+2, r2));
-  q = fma(q, r2, kc(-0x1.3bd3cc9be45dep+2, r2));
+BEX_HD double cospi_tail(double r2, const Coef& C) {
+  double q = C[kCospiTail];
+  q = horner(q, r2, C[kCospiTail + 1]);
+  q = horner(q, r2, C[kCospiTail + 2]);
+  q = horner(q, r2, C[kCospiTail + 3]);
+  q = horner(q, r2, C[kCospiTail + 4]);
+  q = horner(q, r2, C[kCospiTail + 5]);
+  q = horner(q, r2, C[kCospiTail + 6]);
+  q = horner(q, r2, C[kCospiTail + 7]);
+  q = horner(q, r2, C[kCospiTail + 8]);
+  q = horner(q, r2, C[kCospiTail + 9]);
   return r2 * q;
 }
 
 // sin(r) - r and cos(r) - 1 for |r| <= pi/4 (+ reduction slack).
-BEX_HD double sin_tail(double r, double r2) {
-  double p = kc(0x1.71b8ef6dcf572p-66, r2);
-  p = fma(p, r2, kc(-0x1.2f49b46814157p-57, r2));
-  p = fma(p, r2, kc(0x1.952c77030ad4ap-49, r2));
-  p = fma(p, r2, kc(-0x1.ae7f3e733b81fp-41, r2));
-  p = fma(p, r2, kc(0x1.6124613a86d09p-33, r2));
-  p = fma(p, r2, kc(-0x1.ae64567f544e4p-26, r2));
-  p = fma(p, r2, kc(0x1.71de3a556c734p-19, r2));
-  p = fma(p, r2, kc(-0x1.a01a01a01a01ap-13, r2));
-  p = fma(p, r2, kc(0x1.1111111111111p-7, r2));
-  p = fma(p, r2, kc(-0x1.5555555555555p-3, r2));
+BEX_HD double sin_tail(double r, double r2, const Coef& C) {
+  double p = C[kSinTail];
+  p = horner(p, r2, C[kSinTail + 1]);
+  p = horner(p, r2, C[kSinTail + 2]);
+  p = horner(p, r2, C[kSinTail + 3]);
+  p = horner(p, r2, C[kSinTail + 4]);
+  p = horner(p, r2, C[kSinTail + 5]);
+  p = horner(p, r2, C[kSinTail + 6]);
+  p = horner(p, r2, C[kSinTail + 7]);
+  p = horner(p, r2, C[kSinTail + 8]);
+  p = horner(p, r2, C[kSinTail + 9]);
   return (r * r2) * p;
 }
-BEX_HD double cos_tail(double r2) {
-  double q = kc(-0x1.0ce396db7f853p-70, r2);
-  q = fma(q, r2, kc(0x1.e542ba4020225p-62, r2));
-  q = fma(q, r2, kc(-0x1.6827863b97d97p-53, r2));
-  q = fma(q, r2, kc(0x1.ae7f3e733b81fp-45, r2));
-  q = fma(q, r2, kc(-0x1.93974a8c07c9dp-37, r2));
-  q = fma(q, r2, kc(0x1.1eed8eff8d898p-29, r2));
-  q = fma(q, r2, kc(-0x1.27e4fb7789f5cp-22, r2));
-  q = fma(q, r2, kc(0x1.a01a01a01a01ap-16, r2));
-  q = fma(q, r2, kc(-0x1.6c16c16c16c17p-10, r2));
-  q = fma(q, r2, kc(0x1.5555555555555p-5, r2));
-  q = fma(q, r2, kc(-0x1.0000000000000p-1, r2));
+BEX_HD double cos_tail(double r2, const Coef& C) {
+  double q = C[kCosTail];
+  q = horner(q, r2, C[kCosTail + 1]);
+  q = horner(q, r2, C[kCosTail + 2]);
+  q = horner(q, r2, C[kCosTail + 3]);
+  q = horner(q, r2, C[kCosTail + 4]);
+  q = horner(q, r2, C[kCosTail + 5]);
+  q = horner(q, r2, C[kCosTail + 6]);
+  q = horner(q, r2, C[kCosTail + 7]);
+  q = horner(q, r2, C[kCosTail + 8]);
+  q = horner(q, r2, C[kCosTail + 9]);
+  q = horner(q, r2, C[kCosTail + 10]);
   return r2 * q;
 }
 
@@ -123,6 +198,7 @@ BEX_HD double cos_tail(double r2) {
 // sincospi(x) = (sin(pi x), cos(pi x)). The reduction x = n/2 + r is exact, so the result is
 // accurate for every finite x.
 BEX_HD void sincospi(double x, double* s, double* c) {
+  const Coef C(x);
   if (!is_finite(x)) {
     *s = quiet_nan();
     *c = quiet_nan();
@@ -142,8 +218,8 @@ BEX_HD void sincospi(double x, double* s, double* c) {
   int q = (int)q4;
   double r2 = r * r;
   // sin(pi r) = r*pi_hi + (r*pi_lo + tail)
-  double sp = fma(r, kc(0x1.921fb54442d18p+1, r), fma(r, kc(0x1.1a62633145c07p-53, r), sinpi_tail(r, r2)));
-  double cp = 1.0 + cospi_tail(r2);
+  double sp = fma(r, C[kPi], fma(r, C[kPi + 1], sinpi_tail(r, r2, C)));
+  double cp = 1.0 + cospi_tail(r2, C);
   switch (q) {
     case 0: *s = sp; *c = cp; break;
     case 1: *s = cp; *c = -sp; break;
@@ -155,24 +231,25 @@ BEX_HD void sincospi(double x, double* s, double* c) {
 // sin and cos with a three-part Cody-Waite reduction by pi/2. The reduction is exact for
 // |x| < 2^20 * pi/2 (~1.6e6); beyond that the result stays finite but loses accuracy. The
 // solver evaluates these at 2*pi*f2/f1*tau + theta with tau in [0, 1].
-BEX_HD void reduce_pio2(double x, double* r, int* q) {
-  double n = rint(x * kc(0x1.45f306dc9c883p-1, x));
-  double t = fma(-n, kc(0x1.921fb54400000p+0, x), x);
-  t = fma(-n, kc(0x1.0b4611a600000p-34, x), t);
-  t = fma(-n, kc(0x1.3198a2e000000p-69, x), t);
+BEX_HD void reduce_pio2(double x, double* r, int* q, const Coef& C) {
+  double n = rint(x * C[kPio2]);
+  double t = fma(-n, C[kPio2 + 1], x);
+  t = fma(-n, C[kPio2 + 2], t);
+  t = fma(-n, C[kPio2 + 3], t);
   *r = t;
   double q4 = n - 4.0 * floor(n * 0.25);
   *q = (int)q4;
 }
 
 BEX_HD double sin(double x) {
+  const Coef C(x);
   if (!is_finite(x)) return quiet_nan();
   double r;
   int q;
-  reduce_pio2(x, &r, &q);
+  reduce_pio2(x, &r, &q, C);
   double r2 = r * r;
-  double sr = r + sin_tail(r, r2);
-  double cr = 1.0 + cos_tail(r2);
+  double sr = r + sin_tail(r, r2, C);
+  double cr = 1.0 + cos_tail(r2, C);
   switch (q) {
     case 0: return sr;
     case 1: return cr;
@@ -182,13 +259,14 @@ BEX_HD double sin(double x) {
 }
 
 BEX_HD double cos(double x) {
+  const Coef C(x);
   if (!is_finite(x)) return quiet_nan();
   double r;
   int q;
-  reduce_pio2(x, &r, &q);
+  reduce_pio2(x, &r, &q, C);
   double r2 = r * r;
-  double sr = r + sin_tail(r, r2);
-  double cr = 1.0 + cos_tail(r2);
+  double sr = r + sin_tail(r, r2, C);
+  double cr = 1.0 + cos_tail(r2, C);
   switch (q) {
     case 0: return cr;
     case 1: return -sr;
@@ -200,6 +278,7 @@ BEX_HD double cos(double x) {
 // sin and cos of the same argument; bit-identical to sin(x) and cos(x) above (same reduction
 // and polynomials), at the cost of one of them.
 BEX_HD void sincos(double x, double* s, double* c) {
+  const Coef C(x);
   if (!is_finite(x)) {
     *s = quiet_nan();
     *c = quiet_nan();
@@ -207,10 +286,10 @@ BEX_HD void sincos(double x, double* s, double* c) {
   }
   double r;
   int q;
-  reduce_pio2(x, &r, &q);
+  reduce_pio2(x, &r, &q, C);
   double r2 = r * r;
-  double sr = r + sin_tail(r, r2);
-  double cr = 1.0 + cos_tail(r2);
+  double sr = r + sin_tail(r, r2, C);
+  double cr = 1.0 + cos_tail(r2, C);
   switch (q) {
     case 0: *s = sr; *c = cr; break;
     case 1: *s = cr; *c = -sr; break;
@@ -222,6 +301,7 @@ BEX_HD void sincos(double x, double* s, double* c) {
 // ---------------------------------------------------------------------------------------
 // log(x) for finite x > 0, returned as an unevaluated sum hi + lo carrying ~10 extra bits.
 BEX_HD double log_dd(double x, double* lo) {
+  const Coef C(x);
   int k = 0;
   if (x < 0x1.0p-1022) {  // subnormal input
     x *= 0x1.0p54;
@@ -243,30 +323,30 @@ BEX_HD double log_dd(double x, double* lo) {
   double e = fma(-sh, dh, f) - sh * dl;       // f - sh*(dh+dl)
   double sl = e * rinv;
   double s2 = sh * sh;
-  double p = kc(0x1.2f684bda12f68p-5, s2);            // 1/27
-  p = fma(p, s2, kc(0x1.47ae147ae147bp-5, s2));       // 1/25
-  p = fma(p, s2, kc(0x1.642c8590b2164p-5, s2));       // 1/23
-  p = fma(p, s2, kc(0x1.8618618618618p-5, s2));       // 1/21
-  p = fma(p, s2, kc(0x1.af286bca1af28p-5, s2));       // 1/19
-  p = fma(p, s2, kc(0x1.e1e1e1e1e1e1ep-5, s2));       // 1/17
-  p = fma(p, s2, kc(0x1.1111111111111p-4, s2));       // 1/15
-  p = fma(p, s2, kc(0x1.3b13b13b13b14p-4, s2));       // 1/13
-  p = fma(p, s2, kc(0x1.745d1745d1746p-4, s2));       // 1/11
-  p = fma(p, s2, kc(0x1.c71c71c71c71cp-4, s2));       // 1/9
-  p = fma(p, s2, kc(0x1.2492492492492p-3, s2));       // 1/7
-  p = fma(p, s2, kc(0x1.999999999999ap-3, s2));       // 1/5
-  p = fma(p, s2, kc(0x1.5555555555555p-2, s2));       // 1/3
+  double p = C[kLog];            // 1/27
+  p = horner(p, s2, C[kLog + 1]);       // 1/25
+  p = horner(p, s2, C[kLog + 2]);       // 1/23
+  p = horner(p, s2, C[kLog + 3]);       // 1/21
+  p = horner(p, s2, C[kLog + 4]);       // 1/19
+  p = horner(p, s2, C[kLog + 5]);       // 1/17
+  p = horner(p, s2, C[kLog + 6]);       // 1/15
+  p = horner(p, s2, C[kLog + 7]);       // 1/13
+  p = horner(p, s2, C[kLog + 8]);       // 1/11
+  p = horner(p, s2, C[kLog + 9]);       // 1/9
+  p = horner(p, s2, C[kLog + 10]);       // 1/7
+  p = horner(p, s2, C[kLog + 11]);       // 1/5
+  p = horner(p, s2, C[kLog + 12]);       // 1/3
   double tail = (sh * s2) * p;                // s^3 (1/3 + s^2/5 + ...)
   double lm_hi = 2.0 * sh;
   double lm_lo = 2.0 * (sl + tail);
   // + k*ln2, ln2 split so that k*LN2_A is exact.
   double kd = (double)k;
-  double a = kd * kc(0x1.62e42ff000000p-1, kd);
+  double a = kd * C[kLog + 13];
   double bsum = a + lm_hi;
   // two-sum of a and lm_hi
   double bb = bsum - a;
   double err = (a - (bsum - bb)) + (lm_hi - bb);
-  double low = err + (lm_lo + kd * kc(-0x1.718432a1b0e26p-35, kd));
+  double low = err + (lm_lo + kd * C[kLog + 14]);
   double hi = bsum + low;
   *lo = low - (hi - bsum);
   return hi;
@@ -274,24 +354,25 @@ BEX_HD double log_dd(double x, double* lo) {
 
 // exp(th + tl) for |tl| << |th|, th finite.
 BEX_HD double exp_dd(double th, double tl) {
+  const Coef C(th);
   if (th > 709.782712893384) return infinity();
   if (th < -745.1332191019412) return 0.0;
-  double n = rint(th * kc(0x1.71547652b82fep+0, th));
-  double r = fma(-n, kc(0x1.62e42ff000000p-1, th), th);   // exact (Sterbenz region)
-  r = r + (tl - n * kc(-0x1.718432a1b0e26p-35, th));
-  double p = kc(0x1.93974a8c07c9dp-37, r);               // 1/14!
-  p = fma(p, r, kc(0x1.6124613a86d09p-33, r));
-  p = fma(p, r, kc(0x1.1eed8eff8d898p-29, r));
-  p = fma(p, r, kc(0x1.ae64567f544e4p-26, r));
-  p = fma(p, r, kc(0x1.27e4fb7789f5cp-22, r));
-  p = fma(p, r, kc(0x1.71de3a556c734p-19, r));
-  p = fma(p, r, kc(0x1.a01a01a01a01ap-16, r));
-  p = fma(p, r, kc(0x1.a01a01a01a01ap-13, r));
-  p = fma(p, r, kc(0x1.6c16c16c16c17p-10, r));
-  p = fma(p, r, kc(0x1.1111111111111p-7, r));
-  p = fma(p, r, kc(0x1.5555555555555p-5, r));
-  p = fma(p, r, kc(0x1.5555555555555p-3, r));
-  p = fma(p, r, kc(0x1.0000000000000p-1, r));
+  double n = rint(th * C[kExp]);
+  double r = fma(-n, C[kExp + 1], th);   // exact (Sterbenz region)
+  r = r + (tl - n * C[kExp + 2]);
+  double p = C[kExp + 3];               // 1/14!
+  p = horner(p, r, C[kExp + 4]);
+  p = horner(p, r, C[kExp + 5]);
+  p = horner(p, r, C[kExp + 6]);
+  p = horner(p, r, C[kExp + 7]);
+  p = horner(p, r, C[kExp + 8]);
+  p = horner(p, r, C[kExp + 9]);
+  p = horner(p, r, C[kExp + 10]);
+  p = horner(p, r, C[kExp + 11]);
+  p = horner(p, r, C[kExp + 12]);
+  p = horner(p, r, C[kExp + 13]);
+  p = horner(p, r, C[kExp + 14]);
+  p = horner(p, r, C[kExp + 15]);
   double em1 = fma(r * r, p, r);
   double res = 1.0 + em1;
   int ni = (int)n;
