@@ -42,7 +42,8 @@ enum CoefIndex : int {
   kCosTail = 36,
   kLog = 47,
   kExp = 62,
-  kNumCoef = 78
+  kMisc = 78,
+  kNumCoef = 81
 };
 
 #define BEX_DETMATH_COEFFICIENTS \
@@ -83,7 +84,9 @@ enum CoefIndex : int {
     0x1.ae64567f544e4p-26, 0x1.27e4fb7789f5cp-22, 0x1.71de3a556c734p-19, \
     0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-13, 0x1.6c16c16c16c17p-10, \
     0x1.1111111111111p-7, 0x1.5555555555555p-5, 0x1.5555555555555p-3, \
-    0x1.0000000000000p-1
+    0x1.0000000000000p-1, \
+    /* kMisc: sqrt(2) (log reduction); exp overflow and underflow thresholds */ \
+    0x1.6a09e667f3bcdp+0, 709.782712893384, -745.1332191019412
 
 #if defined(__HIP_DEVICE_COMPILE__)
 static __constant__ const double kCoefDevice[kNumCoef] = {BEX_DETMATH_COEFFICIENTS};
@@ -105,16 +108,14 @@ struct Coef {
 };
 
 // One Horner step p * x + c with a table coefficient c, as a single fma. On the device the
-// coefficient stays in its scalar registers as the addend of one v_fma_f64; left to itself the
-// compiler copies it into vector registers for a v_fmac_f64 (three instructions).
+// coefficient is marked as living in scalar registers (an empty asm), which keeps it the addend
+// of one v_fma_f64; left to itself the compiler copies it into vector registers for a
+// v_fmac_f64 (three instructions).
 BEX_HD double horner(double p, double x, double c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double r;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(x), "s"(c));
-  return r;
-#else
-  return fma(p, x, c);
+  asm("" : "+s"(c));
 #endif
+  return fma(p, x, c);
 }
 
 BEX_HD uint64_t as_bits(double x) { return __builtin_bit_cast(uint64_t, x); }
@@ -195,125 +196,104 @@ BEX_HD double cos_tail(double r2, const Coef& C) {
 }
 
 // ---------------------------------------------------------------------------------------
+// The functions below evaluate their main path on sanitised arguments and select the special
+// cases (non-finite input, integer-valued sincospi argument, pow's C99 cases, exp's overflow
+// and underflow) at the end, instead of returning early: on the device an early return is a
+// divergent branch (exec-mask bookkeeping on every call), a select is one instruction per half.
+// The values are those of the same formulas with early returns, for every input.
+
+// Quadrant selection: (s, c) = (sin, cos) of r + q pi/2 from (sin r, cos r) = (sr, cr).
+BEX_HD void quadrant(int q, double sr, double cr, double* s, double* c) {
+  const bool odd = (q & 1) != 0;
+  const double sv = odd ? cr : sr;
+  const double cv = odd ? sr : cr;
+  *s = (q & 2) ? -sv : sv;
+  *c = ((q + 1) & 2) ? -cv : cv;
+}
+
 // sincospi(x) = (sin(pi x), cos(pi x)). The reduction x = n/2 + r is exact, so the result is
 // accurate for every finite x.
 BEX_HD void sincospi(double x, double* s, double* c) {
   const Coef C(x);
-  if (!is_finite(x)) {
-    *s = quiet_nan();
-    *c = quiet_nan();
-    return;
-  }
-  double ax = fabs_d(x);
-  if (ax >= 0x1.0p52) {  // x is an integer: sin = 0, cos = (-1)^x
-    double half = x * 0.5;
-    bool odd = (floor(half) != half);
-    *s = 0.0;
-    *c = odd ? -1.0 : 1.0;
-    return;
-  }
-  double n = rint(2.0 * x);      // exact: 2x is exact, rint is exact
-  double r = fma(-0.5, n, x);    // exact: |r| <= 1/4
-  double q4 = n - 4.0 * floor(n * 0.25);  // n mod 4, exact
-  int q = (int)q4;
-  double r2 = r * r;
+  const double ax = fabs_d(x);
+  const bool finite = is_finite(x);
+  const bool integral = ax >= 0x1.0p52;  // x is an integer: sin = 0, cos = (-1)^x
+  const double xm = (finite && !integral) ? x : 0.0;
+  const double n = rint(2.0 * xm);              // exact: 2x is exact, rint is exact
+  const double r = fma(-0.5, n, xm);            // exact: |r| <= 1/4
+  const double q4 = n - 4.0 * floor(n * 0.25);  // n mod 4, exact
+  const int q = (int)q4;
+  const double r2 = r * r;
   // sin(pi r) = r*pi_hi + (r*pi_lo + tail)
-  double sp = fma(r, C[kPi], fma(r, C[kPi + 1], sinpi_tail(r, r2, C)));
-  double cp = 1.0 + cospi_tail(r2, C);
-  switch (q) {
-    case 0: *s = sp; *c = cp; break;
-    case 1: *s = cp; *c = -sp; break;
-    case 2: *s = -sp; *c = -cp; break;
-    default: *s = -cp; *c = sp; break;
-  }
+  const double sp = fma(r, C[kPi], fma(r, C[kPi + 1], sinpi_tail(r, r2, C)));
+  const double cp = 1.0 + cospi_tail(r2, C);
+  double sv, cv;
+  quadrant(q, sp, cp, &sv, &cv);
+  const double half = x * 0.5;
+  const bool odd = (floor(half) != half);
+  sv = integral ? 0.0 : sv;
+  cv = integral ? (odd ? -1.0 : 1.0) : cv;
+  *s = finite ? sv : quiet_nan();
+  *c = finite ? cv : quiet_nan();
 }
 
 // sin and cos with a three-part Cody-Waite reduction by pi/2. The reduction is exact for
 // |x| < 2^20 * pi/2 (~1.6e6); beyond that the result stays finite but loses accuracy. The
-// solver evaluates these at 2*pi*f2/f1*tau + theta with tau in [0, 1].
+// solver evaluates these at 2*pi*f2/f1*tau + theta with tau in [0, 1]. Non-finite x is
+// replaced by 0 here; the callers return NaN for it.
 BEX_HD void reduce_pio2(double x, double* r, int* q, const Coef& C) {
-  double n = rint(x * C[kPio2]);
-  double t = fma(-n, C[kPio2 + 1], x);
+  const double xm = is_finite(x) ? x : 0.0;
+  const double n = rint(xm * C[kPio2]);
+  double t = fma(-n, C[kPio2 + 1], xm);
   t = fma(-n, C[kPio2 + 2], t);
   t = fma(-n, C[kPio2 + 3], t);
   *r = t;
-  double q4 = n - 4.0 * floor(n * 0.25);
+  const double q4 = n - 4.0 * floor(n * 0.25);
   *q = (int)q4;
 }
 
-BEX_HD double sin(double x) {
+// sin and cos of the same argument.
+BEX_HD void sincos(double x, double* s, double* c) {
   const Coef C(x);
-  if (!is_finite(x)) return quiet_nan();
   double r;
   int q;
   reduce_pio2(x, &r, &q, C);
-  double r2 = r * r;
-  double sr = r + sin_tail(r, r2, C);
-  double cr = 1.0 + cos_tail(r2, C);
-  switch (q) {
-    case 0: return sr;
-    case 1: return cr;
-    case 2: return -sr;
-    default: return -cr;
-  }
+  const double r2 = r * r;
+  const double sr = r + sin_tail(r, r2, C);
+  const double cr = 1.0 + cos_tail(r2, C);
+  double sv, cv;
+  quadrant(q, sr, cr, &sv, &cv);
+  const bool finite = is_finite(x);
+  *s = finite ? sv : quiet_nan();
+  *c = finite ? cv : quiet_nan();
+}
+
+// sin(x) and cos(x): bit-identical to the matching output of sincos.
+BEX_HD double sin(double x) {
+  double s, c;
+  sincos(x, &s, &c);
+  return s;
 }
 
 BEX_HD double cos(double x) {
-  const Coef C(x);
-  if (!is_finite(x)) return quiet_nan();
-  double r;
-  int q;
-  reduce_pio2(x, &r, &q, C);
-  double r2 = r * r;
-  double sr = r + sin_tail(r, r2, C);
-  double cr = 1.0 + cos_tail(r2, C);
-  switch (q) {
-    case 0: return cr;
-    case 1: return -sr;
-    case 2: return -cr;
-    default: return sr;
-  }
-}
-
-// sin and cos of the same argument; bit-identical to sin(x) and cos(x) above (same reduction
-// and polynomials), at the cost of one of them.
-BEX_HD void sincos(double x, double* s, double* c) {
-  const Coef C(x);
-  if (!is_finite(x)) {
-    *s = quiet_nan();
-    *c = quiet_nan();
-    return;
-  }
-  double r;
-  int q;
-  reduce_pio2(x, &r, &q, C);
-  double r2 = r * r;
-  double sr = r + sin_tail(r, r2, C);
-  double cr = 1.0 + cos_tail(r2, C);
-  switch (q) {
-    case 0: *s = sr; *c = cr; break;
-    case 1: *s = cr; *c = -sr; break;
-    case 2: *s = -sr; *c = -cr; break;
-    default: *s = -cr; *c = sr; break;
-  }
+  double s, c;
+  sincos(x, &s, &c);
+  return c;
 }
 
 // ---------------------------------------------------------------------------------------
 // log(x) for finite x > 0, returned as an unevaluated sum hi + lo carrying ~10 extra bits.
 BEX_HD double log_dd(double x, double* lo) {
   const Coef C(x);
-  int k = 0;
-  if (x < 0x1.0p-1022) {  // subnormal input
-    x *= 0x1.0p54;
-    k = -54;
-  }
+  const bool subnormal = x < 0x1.0p-1022;
+  x = subnormal ? x * 0x1.0p54 : x;  // exact
+  int k = subnormal ? -54 : 0;
   uint64_t b = as_bits(x);
   k += (int)((b >> 52) & 0x7ff) - 1023;
   double m = from_bits((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);  // [1, 2)
-  if (m > 0x1.6a09e667f3bcdp+0) {  // m > sqrt(2): move to [sqrt(1/2), sqrt(2))
-    m *= 0.5;
-    k += 1;
-  }
+  const bool above = m > C[kMisc];  // m > sqrt(2): move to [sqrt(1/2), sqrt(2))
+  m = above ? m * 0.5 : m;
+  k += above ? 1 : 0;
   // log(m) = 2 atanh(s), s = f / (2 + f), f = m - 1 (exact).
   double f = m - 1.0;
   double dh = 2.0 + f;
@@ -323,7 +303,7 @@ BEX_HD double log_dd(double x, double* lo) {
   double e = fma(-sh, dh, f) - sh * dl;       // f - sh*(dh+dl)
   double sl = e * rinv;
   double s2 = sh * sh;
-  double p = C[kLog];            // 1/27
+  double p = C[kLog];                   // 1/27
   p = horner(p, s2, C[kLog + 1]);       // 1/25
   p = horner(p, s2, C[kLog + 2]);       // 1/23
   p = horner(p, s2, C[kLog + 3]);       // 1/21
@@ -333,9 +313,9 @@ BEX_HD double log_dd(double x, double* lo) {
   p = horner(p, s2, C[kLog + 7]);       // 1/13
   p = horner(p, s2, C[kLog + 8]);       // 1/11
   p = horner(p, s2, C[kLog + 9]);       // 1/9
-  p = horner(p, s2, C[kLog + 10]);       // 1/7
-  p = horner(p, s2, C[kLog + 11]);       // 1/5
-  p = horner(p, s2, C[kLog + 12]);       // 1/3
+  p = horner(p, s2, C[kLog + 10]);      // 1/7
+  p = horner(p, s2, C[kLog + 11]);      // 1/5
+  p = horner(p, s2, C[kLog + 12]);      // 1/3
   double tail = (sh * s2) * p;                // s^3 (1/3 + s^2/5 + ...)
   double lm_hi = 2.0 * sh;
   double lm_lo = 2.0 * (sl + tail);
@@ -352,14 +332,19 @@ BEX_HD double log_dd(double x, double* lo) {
   return hi;
 }
 
-// exp(th + tl) for |tl| << |th|, th finite.
+// exp(th + tl) for |tl| << |th|: +inf above the overflow threshold, 0 below the underflow
+// threshold, NaN for NaN th.
 BEX_HD double exp_dd(double th, double tl) {
   const Coef C(th);
-  if (th > 709.782712893384) return infinity();
-  if (th < -745.1332191019412) return 0.0;
-  double n = rint(th * C[kExp]);
-  double r = fma(-n, C[kExp + 1], th);   // exact (Sterbenz region)
-  r = r + (tl - n * C[kExp + 2]);
+  const bool over = th > C[kMisc + 1];
+  const bool under = th < C[kMisc + 2];
+  const bool nan = is_nan(th);
+  const bool regular = !(over || under || nan);
+  const double thm = regular ? th : 0.0;
+  const double tlm = regular ? tl : 0.0;
+  double n = rint(thm * C[kExp]);
+  double r = fma(-n, C[kExp + 1], thm);   // exact (Sterbenz region)
+  r = r + (tlm - n * C[kExp + 2]);
   double p = C[kExp + 3];               // 1/14!
   p = horner(p, r, C[kExp + 4]);
   p = horner(p, r, C[kExp + 5]);
@@ -375,45 +360,50 @@ BEX_HD double exp_dd(double th, double tl) {
   p = horner(p, r, C[kExp + 15]);
   double em1 = fma(r * r, p, r);
   double res = 1.0 + em1;
-  int ni = (int)n;
-  // Two-step scaling keeps 2^ni representable for results near the overflow and
-  // underflow thresholds.
-  if (ni > 1000) return ldexp(ldexp(res, 1000), ni - 1000);
-  if (ni < -1000) return ldexp(ldexp(res, -1000), ni + 1000);
-  return ldexp(res, ni);
+  const int ni = (int)n;
+  // Two-step scaling keeps 2^ni representable for results near the overflow and underflow
+  // thresholds; for |ni| <= 1000 the second step is a scaling by 2^0.
+  const int n1 = ni > 1000 ? 1000 : (ni < -1000 ? -1000 : ni);
+  double out = ldexp(ldexp(res, n1), ni - n1);
+  out = over ? infinity() : out;
+  out = under ? 0.0 : out;
+  return nan ? quiet_nan() : out;
 }
 
 BEX_HD bool is_integer(double y) { return rint(y) == y; }
 BEX_HD bool is_odd_integer(double y) {
-  if (!is_integer(y)) return false;
-  if (fabs_d(y) >= 0x1.0p53) return false;
-  double h = y * 0.5;
-  return rint(h) != h;
+  const double h = y * 0.5;
+  return is_integer(y) & (fabs_d(y) < 0x1.0p53) & (rint(h) != h);
 }
 
-// pow(x, y) with the C99 special-case rules.
+// pow(x, y) with the C99 special-case rules (F.9.4.4), in this order of precedence:
+// y = 0 -> 1; x = 1 -> 1; NaN -> NaN; y = +-inf; x < 0 with non-integer y -> NaN; x = +-0;
+// x = +-inf; otherwise exp(y log|x|), negated for x < 0 (or x = -0) and odd integer y.
 BEX_HD double pow(double x, double y) {
-  if (y == 0.0) return 1.0;
-  if (x == 1.0) return 1.0;
-  if (is_nan(x) || is_nan(y)) return quiet_nan();
-  double ax = fabs_d(x);
-  if (!is_finite(y)) {
-    if (ax == 1.0) return 1.0;
-    bool big = ax > 1.0;
-    return (big == (y > 0.0)) ? infinity() : 0.0;
-  }
-  double sign = 1.0;
-  if (x < 0.0 || (x == 0.0 && (as_bits(x) >> 63))) {
-    if (x != 0.0 && is_finite(x) && !is_integer(y)) return quiet_nan();
-    if (is_odd_integer(y)) sign = -1.0;
-  }
-  if (ax == 0.0) return (y > 0.0) ? sign * 0.0 : sign * infinity();
-  if (!is_finite(ax)) return (y > 0.0) ? sign * infinity() : sign * 0.0;
+  const double ax = fabs_d(x);
+  const bool finite_x = is_finite(ax);
+  const bool finite_y = is_finite(y);
+  const bool negative = (as_bits(x) >> 63) != 0;  // x < 0 or x = -0
+  const double sign = (negative && is_odd_integer(y)) ? -1.0 : 1.0;
+  // Main path on a finite, nonzero |x| and a finite y (1 and 0 otherwise).
+  const bool regular = finite_x && (ax != 0.0) && finite_y;
+  const double axm = regular ? ax : 1.0;
+  const double ym = regular ? y : 0.0;
   double ll;
-  double lh = log_dd(ax, &ll);
-  double th = y * lh;
-  double tl = fma(y, lh, -th) + y * ll;
-  return sign * exp_dd(th, tl);
+  const double lh = log_dd(axm, &ll);
+  const double th = ym * lh;
+  const double tl = fma(ym, lh, -th) + ym * ll;
+  double r = sign * exp_dd(th, tl);
+  // Special cases, lowest precedence first.
+  r = !finite_x ? ((y > 0.0) ? sign * infinity() : sign * 0.0) : r;
+  r = (ax == 0.0) ? ((y > 0.0) ? sign * 0.0 : sign * infinity()) : r;
+  r = (negative && (x != 0.0) && finite_x && !is_integer(y)) ? quiet_nan() : r;
+  const double at_inf = (ax == 1.0) ? 1.0 : (((ax > 1.0) == (y > 0.0)) ? infinity() : 0.0);
+  r = !finite_y ? at_inf : r;
+  r = (is_nan(x) || is_nan(y)) ? quiet_nan() : r;
+  r = (x == 1.0) ? 1.0 : r;
+  r = (y == 0.0) ? 1.0 : r;
+  return r;
 }
 
 }  // namespace bex

@@ -209,6 +209,16 @@ struct VmaxOp {
   }
 };
 
+// min(a, b) in one v_min_f64 (IEEE minNum), for operands that are > 0 and not NaN: equal to
+// MinKeepOp there, and commutative.
+struct VminOp {
+  __device__ __forceinline__ double operator()(double a, double b) const {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+};
+
 struct AddOp {
   __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
 };
@@ -813,13 +823,11 @@ struct RolePairLayout32 {
   // Both halves hold identical per-unit values, so wave-wide AND/OR/min equal the system's.
   __device__ __forceinline__ bool all(bool p) const { return __ballot(!p) == 0ull; }
   __device__ __forceinline__ bool any(bool p) const { return __ballot(p) != 0ull; }
+  // Minimum over the system's units, broadcast as a wave-uniform value. Its operands (the
+  // controller's tol/err ratios, the event step candidates) are positive and never NaN, where
+  // v_min_f64 and the keep-on-NaN minimum agree.
   __device__ __forceinline__ double min_keep(double v) const {
-#pragma unroll
-    for (int h = 16; h >= 1; h >>= 1) {
-      const double o = __shfl_xor(v, h, kWave);
-      v = (o < v) ? o : v;
-    }
-    return uniform(v);
+    return uniform(tree32(v, VminOp()));
   }
 
   // acc_i = sum_j mrow[j] * z_j over j = 0..31 in the canonical order (four partial sums),

@@ -63,6 +63,50 @@ def test_detmath_pow_special_values(oracle):
     assert oracle.pow_(2.0, 10.0) == 1024.0
 
 
+def test_detmath_pow_special_case_grid(oracle):
+    """Every C99 special case (F.9.4.4) against the C library's pow (numpy.power), including
+    signed zeros, infinities, NaN, negative bases with integer and non-integer exponents and a
+    subnormal base; the regular pairs of the grid agree within 2 ulp with the same signs."""
+    xs = [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, -3.0, math.inf, -math.inf, math.nan,
+          1e-310, -1e-310, 1e300]
+    ys = [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 3.0, -3.0, 0.5, -0.5, 4.2, math.inf, -math.inf,
+          math.nan, 1e300, -1e300]
+    with np.errstate(all="ignore"):
+        for x in xs:
+            for y in ys:
+                got = oracle.pow_(x, y)
+                want = float(np.power(np.float64(x), np.float64(y)))
+                tag = (x, y, got, want)
+                if math.isnan(want):
+                    assert math.isnan(got), tag
+                elif want == 0.0 or math.isinf(want):
+                    assert got == want and math.copysign(1.0, got) == math.copysign(1.0, want), tag
+                else:
+                    assert math.copysign(1.0, got) == math.copysign(1.0, want), tag
+                    assert ulp_err(got, mp.mpf(want)) <= 2.0, tag
+
+
+def test_detmath_trig_special_inputs(oracle):
+    for x in (math.inf, -math.inf, math.nan):
+        s, c = oracle.sincospi(x)
+        assert math.isnan(s) and math.isnan(c)
+        assert math.isnan(oracle.sin_(x)) and math.isnan(oracle.cos_(x))
+    # Integer arguments beyond 2^52: sin(pi x) = +0, cos(pi x) = (-1)^x.
+    for x, c_want in ((2.0 ** 52 + 1.0, -1.0), (-(2.0 ** 52) - 1.0, -1.0), (2.0 ** 60, 1.0)):
+        s, c = oracle.sincospi(x)
+        assert s == 0.0 and math.copysign(1.0, s) == 1.0 and c == c_want
+    # Quadrant bookkeeping on both sides of each multiple of pi/2 (sincos) and of 1/2
+    # (sincospi) against mpmath.
+    for k in range(-8, 9):
+        for d in (-1e-3, 1e-3):
+            x = k * 0.5 + d
+            s, c = oracle.sincospi(x)
+            assert ulp_err(s, mp.sinpi(mp.mpf(x))) <= 2.0 and ulp_err(c, mp.cospi(mp.mpf(x))) <= 2.0
+            y = k * math.pi / 2 + d
+            assert ulp_err(oracle.sin_(y), mp.sin(mp.mpf(y))) <= 2.0
+            assert ulp_err(oracle.cos_(y), mp.cos(mp.mpf(y))) <= 2.0
+
+
 def test_detmath_sincospi_accuracy(oracle):
     rng = np.random.default_rng(2)
     xs = np.concatenate([rng.uniform(-8, 8, 600), [0.0, 0.25, 0.5, 1.0, 1.5, -0.75, 2.0 + 1e-9]])
