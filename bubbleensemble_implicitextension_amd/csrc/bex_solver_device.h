@@ -101,7 +101,7 @@ struct BicgOut {
 
 // acc[j % 4] = fma(z(lane j of the lane's 16-lane row), m[j], acc[j % 4]) for j = 0..15, in
 // order: the canonical four-partial-sum order of every matrix-vector product. The s_nop covers
-// the DPP read-after-VALU-write hazard of z.
+// the DPP read-after-VALU-write hazard of z (the compiler does not look inside the asm).
 __device__ __forceinline__ void fmac_row16(double* acc, double z, const double* m) {
   asm volatile(
       "s_nop 1\n"
@@ -195,6 +195,28 @@ __device__ __forceinline__ void tree32x2(double& a, double& b, OpA opa, OpB opb)
   a = opa(a, row_ror<1>(a));
   b = opb(b, row_ror<1>(b));
 }
+
+// The levels h = 8, 4, 2, 1 of two trees (as tree32x2), in lockstep.
+template <class OpA, class OpB>
+__device__ __forceinline__ void tree16x2(double& a, double& b, OpA opa, OpB opb) {
+  a = opa(a, row_ror<8>(a));
+  b = opb(b, row_ror<8>(b));
+  a = opa(a, row_ror<4>(a));
+  b = opb(b, row_ror<4>(b));
+  a = opa(a, row_ror<2>(a));
+  b = opb(b, row_ror<2>(b));
+  a = opa(a, row_ror<1>(a));
+  b = opb(b, row_ror<1>(b));
+}
+
+// max(|a|, |b|) in one v_max_f64 with absolute-value source modifiers (as VmaxOp on |a|, |b|).
+struct VmaxAbsOp {
+  __device__ __forceinline__ double operator()(double a, double b) const {
+    double r;
+    asm("v_max_f64 %0, |%1|, |%2|" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  }
+};
 
 // max(a, b) for operands that are >= +0 or a quiet NaN, in one v_max_f64 (IEEE maxNum: a NaN
 // operand yields the other one). On such operands it equals bex::fmax_d (the reference's fmax,
@@ -885,12 +907,17 @@ struct RolePairLayout32 {
           res = res - alpha * qv;
           const double rho2 = rho1;
           // The residual test (max |r| over the primal half, evaluated on both halves) and the
-          // next r . r~ are independent: reduce them together.
+          // next r . r~ are independent: reduce them together. Their first level (h = 16) is
+          // formed from the two rows of r and r~: max(|r_c|, |r_c+16|) and
+          // r_c r~_c + r_c+16 r~_c+16, the same operations as on |r| and the products.
           double lo2, up2;
-          halves(res, &lo2, &up2);
-          double rmax = bex::fabs_d(lo2);
-          rho1 = lo2 * up2;
-          tree32x2(rmax, rho1, VmaxOp(), AddOp());
+          halves(res, &lo2, &up2);  // r and r~ of the lane's unit, on both halves
+          double re, ro, te, to;
+          rows_of_half(lo2, &re, &ro);
+          rows_of_half(up2, &te, &to);
+          double rmax = VmaxAbsOp()(re, ro);
+          rho1 = re * te + ro * to;
+          tree16x2(rmax, rho1, VmaxOp(), AddOp());
           out.rmax = rmax;
           if (any(rmax < a.BiCGResidualTolerance)) {
             out.maxit = false;
@@ -981,6 +1008,10 @@ struct RolePairLayout32 {
     return out;
   }
 };
+
+// Adds v to the system's counter k in LDS. An LDS atomic whose result is unused needs no wait,
+// where a read-modify-write would stall the wave for the LDS round trip.
+__device__ __forceinline__ void count(int* st, int k, int v) { atomicAdd(&st[k], v); }
 
 // =========================================================================================
 // Time loop of one system (Solver.cuh:11-651), shared by the layouts.
@@ -1153,7 +1184,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       dt = TD1 - t;
       endReached = 1;
     }
-    if (leader && S.flushOnly == 0) st[BEX_STAT_ATTEMPTS]++;
+    if (leader && S.flushOnly == 0) count(st, BEX_STAT_ATTEMPTS, 1);
 
     bool finiteLane = true;
     double k1[UD], k2[UD], k3[UD], k4[UD], k5[UD], k6[UD];
@@ -1227,11 +1258,11 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       };
       const BicgOut o = lay.coupling(F, xs[0], cpt, cpf, a, features, sample_now);
       if (leader) {
-        st[BEX_STAT_STAGES]++;
-        st[BEX_STAT_BICG_ITERATIONS] += o.iterations;
-        st[BEX_STAT_BICG_RHO_EXITS] += o.rho_exit ? 1 : 0;
-        st[BEX_STAT_BICG_MAXIT] += o.maxit ? 1 : 0;
-        st[BEX_STAT_HIST0 + (o.iterations < 20 ? o.iterations : 19)]++;
+        count(st, BEX_STAT_STAGES, 1);
+        count(st, BEX_STAT_BICG_ITERATIONS, o.iterations);
+        count(st, BEX_STAT_BICG_RHO_EXITS, o.rho_exit ? 1 : 0);
+        count(st, BEX_STAT_BICG_MAXIT, o.maxit ? 1 : 0);
+        count(st, BEX_STAT_HIST0 + (o.iterations < 20 ? o.iterations : 19), 1);
       }
 #pragma unroll
       for (int i = 0; i < UD; i++) {
@@ -1373,7 +1404,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       if (accepted == 2 * a.PriorityStepThreshold) __builtin_amdgcn_s_setprio(2);
       if (accepted == 4 * a.PriorityStepThreshold) __builtin_amdgcn_s_setprio(3);
       if (uid == 0) {
-        if (leader) st[BEX_STAT_ACCEPTED]++;
+        if (leader) count(st, BEX_STAT_ACCEPTED, 1);
         S.T = t;
         S.dT = dt;
       }
@@ -1411,7 +1442,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         lay.sync();
       }
     } else if (leader) {
-      st[BEX_STAT_REJECTED]++;
+      count(st, BEX_STAT_REJECTED, 1);
     }
     const int udt = S.UDT;
 
