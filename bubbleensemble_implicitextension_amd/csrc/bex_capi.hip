@@ -133,10 +133,16 @@ KernelChoice choice(int alg, bool dense) {
   return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
 }
 
-// layout: BEX_LAYOUT_AUTO picks the role-pair layout for 32 units and the group layout
-// otherwise. mode: bexk::WarmStartMode; the learned guess and sample collection exist for the
-// role-pair layout only.
+// layout: BEX_LAYOUT_AUTO picks the role-pair layouts (32 units: one system per wave; 128
+// units: ChainLayout128, the matrix in registers) and the lane-per-unit layouts otherwise;
+// BEX_LAYOUT_GROUP forces lane-per-unit (for 128 units: WideLayout, the matrix in LDS). mode:
+// bexk::WarmStartMode; the learned guess and sample collection exist for 32 units only.
 bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice* k) {
+  if (ups == 128 && layout != BEX_LAYOUT_GROUP) {
+    if (mode != bexk::kPlain) return false;
+    *k = choice<bexk::ChainLayout128<false>>(alg, dense);
+    return true;
+  }
   if (ups == 32 && layout != BEX_LAYOUT_GROUP) {
     switch (mode) {
       case bexk::kPlain: *k = choice<bexk::RolePairLayout32<bexk::kPlain>>(alg, dense); return true;
@@ -1097,6 +1103,7 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<64, 1, true>>("group64 x1 wave"),
       pool_variant<bexk::GroupLayout<64, 2, true>>("group64 x2 waves")};
   static const std::vector<PoolVariant> v128 = {
+      pool_variant<bexk::ChainLayout128<true>>("chain128 x8 waves, M in registers"),
       pool_variant<bexk::WideLayout<128, true, 2>>("wide128 lane pairs x4 waves"),
       pool_variant<bexk::WideLayout<128, true, 1>>("wide128 x2 waves")};
   static const std::vector<PoolVariant> v256 = {

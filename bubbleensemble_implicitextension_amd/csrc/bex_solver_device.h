@@ -768,6 +768,69 @@ __device__ __forceinline__ float mlp_factor(const BexMlpDevice* net, const doubl
   return net->b3 + (p0 + p1);
 }
 
+// BiCG of the role-pair layouts, in which the primal half of a wave carries r, p, q, x of its
+// units and the shadow half r~, p~, q~ in the same registers (one instruction stream serves
+// A p and A^T p~). The layout L provides matvec(z) (M z on the primal half, M^T z on the shadow
+// half, for a per-unit z), dot_halves(v) (sum over the units of primal(v) * shadow(v), on
+// every lane), res_trees(r, &max|r|, &r.r~), uniform_any(p) and primal_half().
+
+// Initial residual r0 = b - A x0 of the primal half, on both halves.
+template <class L>
+__device__ __forceinline__ double rp_residual(const L& lay, double b, double x0, double x1sq,
+                                              double rD) {
+  const double ax = fma(rD, lay.matvec(x1sq * x0), x0);
+  double lo, up;
+  halves(-ax + b, &lo, &up);
+  return lo;
+}
+
+// BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
+// meaningful on the primal half. The loop is rotated: the reference's iteration-1 branch (the
+// rho exit, p = r) runs before it and the direction update p = r + beta p of iteration k + 1
+// closes iteration k, which leaves the arithmetic and its order unchanged. Scalars (rho,
+// alpha, beta, max |r|) are identical in every lane, so the branches on them are uniform.
+template <class L>
+__device__ __forceinline__ BicgOut rp_bicg(const L& lay, double res0, double x0, double x1sq,
+                                           double rD, const BexKernelArgs& a, double* xout) {
+  const bool primal = lay.primal_half();
+  double res = res0;                     // r on the primal half, r~ = r on the shadow half
+  const double s1 = primal ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
+  const double s2 = primal ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
+  double x = x0;
+  BicgOut out{0, false, true, bex::infinity()};
+  const int maxit = a.BiCGMaxIterations;
+  if (maxit >= 1) {
+    double rho1 = lay.dot_halves(res);  // r . r~
+    if (lay.uniform_any(bex::fabs_d(rho1) < a.BiCGRhoTolerance)) {
+      out.rho_exit = true;
+      out.maxit = false;
+    } else {
+      double dir = res;  // p (primal), p~ (shadow)
+      for (int iter = 1;; iter++) {
+        const double qv = fma(s2, lay.matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
+        out.iterations++;
+        const double alpha = rho1 / lay.dot_halves(primal ? qv : dir);  // p~ . q
+        x = x + alpha * dir;
+        res = res - alpha * qv;
+        const double rho2 = rho1;
+        // The residual test and the next r . r~ are independent: reduced together.
+        double rmax;
+        lay.res_trees(res, &rmax, &rho1);
+        out.rmax = rmax;
+        if (lay.uniform_any(rmax < a.BiCGResidualTolerance)) {
+          out.maxit = false;
+          break;
+        }
+        if (iter >= maxit) break;
+        const double beta = rho1 / rho2;
+        dir = res + beta * dir;
+      }
+    }
+  }
+  *xout = x;
+  return out;
+}
+
 template <int MODE, bool PS = false>
 struct RolePairLayout32 {
   static constexpr int kUnits = 32;
@@ -840,6 +903,7 @@ struct RolePairLayout32 {
   __device__ __forceinline__ int sid() const { return sid_; }
   __device__ __forceinline__ int uid() const { return lane & 31; }
   __device__ __forceinline__ bool primal() const { return primal_; }
+  __device__ __forceinline__ bool primal_half() const { return primal_; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 
   // Both halves hold identical per-unit values, so wave-wide AND/OR/min equal the system's.
@@ -871,66 +935,30 @@ struct RolePairLayout32 {
     return tree32(lo * up, AddOp());
   }
 
-  // Initial residual r0 = b - A x0 of the primal half, on both halves.
-  __device__ __forceinline__ double residual(double b, double x0, double x1sq, double rD) const {
-    const double ax = fma(rD, matvec(x1sq * x0), x0);
-    double lo, up;
-    halves(-ax + b, &lo, &up);
-    return lo;
+  // The residual test's max |r| over the primal half and r . r~, on every lane of both halves.
+  // Their first level (h = 16) is formed from the two rows of r and r~: max(|r_c|, |r_c+16|) and
+  // r_c r~_c + r_c+16 r~_c+16, the same operations as on |r| and the products.
+  __device__ __forceinline__ void res_trees(double res, double* rmax, double* rho) const {
+    double lo2, up2;
+    halves(res, &lo2, &up2);  // r and r~ of the lane's unit, on both halves
+    double re, ro, te, to;
+    rows_of_half(lo2, &re, &ro);
+    rows_of_half(up2, &te, &to);
+    double m = VmaxAbsOp()(re, ro);
+    double p = re * te + ro * to;
+    tree16x2(m, p, VmaxOp(), AddOp());
+    *rmax = m;
+    *rho = p;
   }
+  // Branch on a predicate that has the same value in every lane.
+  __device__ __forceinline__ bool uniform_any(bool p) const { return any(p); }
 
-  // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
-  // meaningful on the primal half. The loop is rotated: the reference's iteration-1 branch (the
-  // rho exit, p = r) runs before it and the direction update p = r + beta p of iteration k + 1
-  // closes iteration k, which leaves the arithmetic and its order unchanged. Scalars (rho,
-  // alpha, beta, max |r|) are identical in every lane, so the branches on them are wave-uniform.
+  __device__ __forceinline__ double residual(double b, double x0, double x1sq, double rD) const {
+    return rp_residual(*this, b, x0, x1sq, rD);
+  }
   __device__ __forceinline__ BicgOut bicg(double res0, double x0, double x1sq, double rD,
                                           const BexKernelArgs& a, double* xout) const {
-    double res = res0;  // r on the primal half, r~ = r on the shadow half
-    const double s1 = primal_ ? x1sq : rD;  // A p    = p  + rD   (M   (x1^2 p))
-    const double s2 = primal_ ? rD : x1sq;  // A^T p~ = p~ + x1^2 (M^T (rD   p~))
-    double x = x0;
-    BicgOut out{0, false, true, bex::infinity()};
-    const int maxit = a.BiCGMaxIterations;
-    if (maxit >= 1) {
-      double rho1 = dot_halves(res);  // r . r~
-      if (any(bex::fabs_d(rho1) < a.BiCGRhoTolerance)) {
-        out.rho_exit = true;
-        out.maxit = false;
-      } else {
-        double dir = res;  // p (primal), p~ (shadow)
-        for (int iter = 1;; iter++) {
-          const double qv = fma(s2, matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
-          out.iterations++;
-          const double alpha = rho1 / dot_halves(primal_ ? qv : dir);  // p~ . q
-          x = x + alpha * dir;
-          res = res - alpha * qv;
-          const double rho2 = rho1;
-          // The residual test (max |r| over the primal half, evaluated on both halves) and the
-          // next r . r~ are independent: reduce them together. Their first level (h = 16) is
-          // formed from the two rows of r and r~: max(|r_c|, |r_c+16|) and
-          // r_c r~_c + r_c+16 r~_c+16, the same operations as on |r| and the products.
-          double lo2, up2;
-          halves(res, &lo2, &up2);  // r and r~ of the lane's unit, on both halves
-          double re, ro, te, to;
-          rows_of_half(lo2, &re, &ro);
-          rows_of_half(up2, &te, &to);
-          double rmax = VmaxAbsOp()(re, ro);
-          rho1 = re * te + ro * to;
-          tree16x2(rmax, rho1, VmaxOp(), AddOp());
-          out.rmax = rmax;
-          if (any(rmax < a.BiCGResidualTolerance)) {
-            out.maxit = false;
-            break;
-          }
-          if (iter >= maxit) break;
-          const double beta = rho1 / rho2;
-          dir = res + beta * dir;
-        }
-      }
-    }
-    *xout = x;
-    return out;
+    return rp_bicg(*this, res0, x0, x1sq, rD, a, xout);
   }
 
   // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT) (Steppers.cuh:489-561).
@@ -1005,6 +1033,250 @@ struct RolePairLayout32 {
         row[9] = 0.0f;
       }
     }
+    return out;
+  }
+};
+
+// =========================================================================================
+// ChainLayout128: one N = 128 system per workgroup of eight waves, the coupling matrix in
+// registers. Wave w serves the 32 units of block B = w / 2 as a role pair (primal half: M and
+// r, p, q, x; shadow half: M^T and r~, p~, q~) and runs two of the four FMA chains of every row
+// product, chains 2 (w % 2) and 2 (w % 2) + 1, over all 128 columns: lane k of the half holds
+// row u = 32 B + k of M (or M^T) at the chains' 64 columns, 128 VGPRs. A product's operand is
+// redistributed once through LDS so that lane k of every half holds z_(4k + q) of its two
+// chains; each chain then runs as 32 v_fmac_f64 with that operand broadcast by DPP
+// row_newbcast from lane t, in column order. The two chain pairs of a row meet in LDS and are
+// added as (s0 + s1) + (s2 + s3), the canonical order; reductions over the 128 units take their
+// first two halving levels (h = 64, 32) from LDS and the rest in-wave. Every wave evaluates the
+// BiCG scalars identically, so its branches agree across the workgroup without a barrier.
+// Compared with M in LDS (WideLayout<128>), a BiCG iteration reads ~12 KB of LDS instead of
+// ~512 KB.
+
+__device__ __forceinline__ void fmac2_row8_lo(double& sa, double& sb, double za, double zb,
+                                             const double* ma, const double* mb) {
+  asm volatile(
+      "s_nop 1\n"
+      "v_fmac_f64_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %12 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %13 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %14 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %15 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %16 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %18 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %19 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(sa), "+v"(sb)
+      : "v"(za), "v"(zb), "v"(ma[0]), "v"(ma[1]), "v"(ma[2]), "v"(ma[3]), "v"(ma[4]), "v"(ma[5]), "v"(ma[6]), "v"(ma[7]),
+        "v"(mb[0]), "v"(mb[1]), "v"(mb[2]), "v"(mb[3]), "v"(mb[4]), "v"(mb[5]), "v"(mb[6]), "v"(mb[7]));
+}
+
+__device__ __forceinline__ void fmac2_row8_hi(double& sa, double& sb, double za, double zb,
+                                             const double* ma, const double* mb) {
+  asm volatile(
+      "s_nop 1\n"
+      "v_fmac_f64_dpp %0, %2, %4 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %5 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %6 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %7 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %8 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %16 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %9 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %10 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %18 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %2, %11 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %3, %19 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(sa), "+v"(sb)
+      : "v"(za), "v"(zb), "v"(ma[0]), "v"(ma[1]), "v"(ma[2]), "v"(ma[3]), "v"(ma[4]), "v"(ma[5]), "v"(ma[6]), "v"(ma[7]),
+        "v"(mb[0]), "v"(mb[1]), "v"(mb[2]), "v"(mb[3]), "v"(mb[4]), "v"(mb[5]), "v"(mb[6]), "v"(mb[7]));
+}
+
+template <bool PS = true>
+struct ChainLayout128 {
+  static constexpr int N = 128;
+  static constexpr int kUnits = N;
+  static constexpr int kWaves = 8;
+  static constexpr int kBlock = kWaves * kWave;
+  static constexpr int kSystemsPerBlock = 1;
+  static constexpr int kWavesPerEU = 2;  // 128 VGPRs of matrix per lane: 256-register budget
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 1;
+
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * (2 * N + 8 * 32 * 2 + N + N) + sizeof(double2) * N + sizeof(Sys) +
+           sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
+  }
+
+  int tid, hh, k, blk, qp, u, sid_;
+  double ma[32], mb[32];  // row u of M (primal) or of M^T (shadow) at columns 4t + 2 qp (+ 1)
+  double* zbuf;           // [2][N]: product operands, primal then shadow
+  double* cc;             // [4 blocks][2 halves][32][2 chain pairs]
+  double* red;            // [32][4 blocks]
+  double2* red2;          // [32][4 blocks]
+  double* redm;           // [32][4 blocks]
+  Sys* sys;
+  int* stat;
+  int* flags;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs&) {
+    tid = threadIdx.x;
+    const int w = tid / kWave, l = tid & (kWave - 1);
+    hh = l >> 5;
+    k = l & 31;
+    blk = w >> 1;
+    qp = w & 1;
+    u = 32 * blk + k;
+    double* p = reinterpret_cast<double*>(smem);
+    zbuf = p;
+    cc = zbuf + 2 * N;
+    red = cc + 8 * 32 * 2;
+    redm = red + N;
+    red2 = reinterpret_cast<double2*>(redm + N);
+    sys = reinterpret_cast<Sys*>(red2 + N);
+    stat = reinterpret_cast<int*>(sys + 1);
+    flags = stat + BEX_NSTAT;
+  }
+
+  // Work item = one system. Loads the lane's 64 matrix entries, element (u, j) of M (primal) or
+  // of M^T (shadow). A shared matrix comes with its transpose (both column-major: coalesced
+  // across the lanes of a half); per-system matrices come without one, so the shadow half reads
+  // column u of M itself (strided, once per system).
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    int s = -1;
+    if (item < count) s = order ? order[item] : a.first_system + (int)item;
+    sid_ = s;
+    if (tid < BEX_NSTAT) stat[tid] = 0;
+    if (s >= 0 && s < a.NS) {
+      const double* m;
+      size_t su, sj;  // strides of u and j
+      if (PS) {
+        m = a.CouplingMatrix + (size_t)s * N * N;
+        su = hh == 0 ? 1 : N;
+        sj = hh == 0 ? N : 1;
+      } else {
+        m = hh == 0 ? a.CouplingMatrix : a.CouplingMatrixT;
+        su = 1;
+        sj = N;
+      }
+      const double* src = m + (size_t)u * su;
+#pragma unroll
+      for (int t = 0; t < 32; t++) {
+        ma[t] = src[(size_t)(4 * t + 2 * qp) * sj];
+        mb[t] = src[(size_t)(4 * t + 2 * qp + 1) * sj];
+      }
+    }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return u; }
+  __device__ __forceinline__ bool primal() const { return hh == 0 && qp == 0; }
+  __device__ __forceinline__ bool primal_half() const { return hh == 0; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+
+  __device__ __forceinline__ bool all(bool p) const {
+    if (tid == 0) flags[0] = 0;
+    __syncthreads();
+    if (!p) flags[0] = 1;
+    __syncthreads();
+    const bool r = flags[0] == 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool any(bool p) const {
+    if (tid == 0) flags[1] = 0;
+    __syncthreads();
+    if (p) flags[1] = 1;
+    __syncthreads();
+    const bool r = flags[1] != 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool uniform_any(bool p) const { return __ballot(p) != 0ull; }
+
+  // Minimum over the units (positive, non-NaN operands: see RolePairLayout32::min_keep).
+  __device__ __forceinline__ double min_keep(double v) const {
+    if (primal()) redm[k * 4 + blk] = v;
+    __syncthreads();
+    const double y = VminOp()(VminOp()(redm[k * 4], redm[k * 4 + 2]),
+                              VminOp()(redm[k * 4 + 1], redm[k * 4 + 3]));
+    const double r = uniform(tree32(y, VminOp()));
+    __syncthreads();
+    return r;
+  }
+
+  // M z (primal half) and M^T z (shadow half) for the per-unit operand z, canonical order.
+  __device__ __forceinline__ double matvec(double z) const {
+    if (qp == 0) zbuf[hh * N + u] = z;
+    __syncthreads();
+    const double2 zz = reinterpret_cast<const double2*>(zbuf + hh * N)[2 * k + qp];
+    double ea, oa, eb, ob;
+    rows_of_half(zz.x, &ea, &oa);  // z_(4t + 2qp) from lane t of the half
+    rows_of_half(zz.y, &eb, &ob);  // z_(4t + 2qp + 1)
+    double sa = 0.0, sb = 0.0;
+    fmac2_row8_lo(sa, sb, ea, eb, &ma[0], &mb[0]);
+    fmac2_row8_hi(sa, sb, ea, eb, &ma[8], &mb[8]);
+    fmac2_row8_lo(sa, sb, oa, ob, &ma[16], &mb[16]);
+    fmac2_row8_hi(sa, sb, oa, ob, &ma[24], &mb[24]);
+    cc[((blk * 2 + hh) * 32 + k) * 2 + qp] = sa + sb;  // s0 + s1, or s2 + s3
+    __syncthreads();
+    const double2 both = reinterpret_cast<const double2*>(cc)[(blk * 2 + hh) * 32 + k];
+    return both.x + both.y;
+  }
+
+  // Sum over the 128 units of primal(v) * shadow(v): levels h = 64, 32 from the four blocks'
+  // products in LDS, then the in-wave tree; on every lane.
+  __device__ __forceinline__ double dot_halves(double v) const {
+    double lo, up;
+    halves(v, &lo, &up);
+    if (primal()) red[k * 4 + blk] = lo * up;
+    __syncthreads();
+    const double2 v01 = reinterpret_cast<const double2*>(red)[2 * k];
+    const double2 v23 = reinterpret_cast<const double2*>(red)[2 * k + 1];
+    return tree32((v01.x + v23.x) + (v01.y + v23.y), AddOp());
+  }
+
+  // max |r| and r . r~ over the 128 units, on every lane.
+  __device__ __forceinline__ void res_trees(double res, double* rmax, double* rho) const {
+    double lo2, up2;
+    halves(res, &lo2, &up2);
+    if (primal()) red2[k * 4 + blk] = make_double2(bex::fabs_d(lo2), lo2 * up2);
+    __syncthreads();
+    const double2 w0 = red2[k * 4], w1 = red2[k * 4 + 1], w2 = red2[k * 4 + 2],
+                  w3 = red2[k * 4 + 3];
+    double m = VmaxOp()(VmaxOp()(w0.x, w2.x), VmaxOp()(w1.x, w3.x));
+    double p = (w0.y + w2.y) + (w1.y + w3.y);
+    tree32x2(m, p, VmaxOp(), AddOp());
+    *rmax = m;
+    *rho = p;
+  }
+
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    const double b = fma(rD, matvec(cpt), xu);  // meaningful on the primal half
+    const double r0 = rp_residual(*this, b, xu, x1sq, rD);
+    double x;
+    const BicgOut out = rp_bicg(*this, r0, xu, x1sq, rD, a, &x);
+    double lo, up;
+    halves(x, &lo, &up);
+    F[1] = lo;
     return out;
   }
 };

@@ -29,12 +29,14 @@ def test_small_ensembles_bitexact(oracle, N, layout):
     s.close()
 
 
-@pytest.mark.parametrize("N,end", [(128, 0.2), (256, 0.05)])
-def test_wide_layouts_bitexact(oracle, N, end):
-    """Shared coupling matrix, multi-wave systems: N = 128 (M in LDS), N = 256 (M streamed)."""
+@pytest.mark.parametrize("N,end,layout", [(128, 0.2, LAYOUT_ROLE_PAIR), (128, 0.2, LAYOUT_GROUP),
+                                           (256, 0.05, LAYOUT_GROUP)])
+def test_wide_layouts_bitexact(oracle, N, end, layout):
+    """Shared coupling matrix, multi-wave systems: N = 128 with the matrix in registers (eight
+    role-pair waves, two FMA chains each) or in LDS (lane per unit), N = 256 (M streamed)."""
     ens = wl.small_ensemble(n=N, ns=3, seed=3 + N)
-    print(f"small ensemble N={N} NS={ens.NS} end={end}", flush=True)
-    s = run_gpu(ens, periods=1, end_time=end)
+    print(f"small ensemble N={N} NS={ens.NS} end={end} layout={layout}", flush=True)
+    s = run_gpu(ens, periods=1, end_time=end, layout=layout)
     p = run_oracle(oracle, ens, periods=1, end_time=end)
     compare_solver_to_oracle(s, p)
     s.close()
