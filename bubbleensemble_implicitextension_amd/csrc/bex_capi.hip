@@ -133,14 +133,17 @@ KernelChoice choice(int alg, bool dense) {
   return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
 }
 
-// layout: BEX_LAYOUT_AUTO picks the role-pair layouts (32 units: one system per wave; 128
-// units: ChainLayout128, the matrix in registers) and the lane-per-unit layouts otherwise;
-// BEX_LAYOUT_GROUP forces lane-per-unit (for 128 units: WideLayout, the matrix in LDS). mode:
+// layout: BEX_LAYOUT_AUTO picks the role-pair layouts (32 units: one system per wave; 64 and
+// 128 units: two and eight waves per system, the matrix in registers) and the lane-per-unit
+// layouts otherwise; BEX_LAYOUT_GROUP forces lane-per-unit (the matrix in LDS). mode:
 // bexk::WarmStartMode; the learned guess and sample collection exist for 32 units only.
 bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice* k) {
-  if (ups == 128 && layout != BEX_LAYOUT_GROUP) {
+  if ((ups == 64 || ups == 128) && layout != BEX_LAYOUT_GROUP) {
     if (mode != bexk::kPlain) return false;
-    *k = choice<bexk::ChainLayout128<false>>(alg, dense);
+    if (ups == 64)
+      *k = choice<bexk::RolePairLayout64<false>>(alg, dense);
+    else
+      *k = choice<bexk::ChainLayout128<false>>(alg, dense);
     return true;
   }
   if (ups == 32 && layout != BEX_LAYOUT_GROUP) {
@@ -1100,6 +1103,7 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<32, 1, true>>("group32 x1 wave"),
       pool_variant<bexk::GroupLayout<32, 2, true>>("group32 x2 waves")};
   static const std::vector<PoolVariant> v64 = {
+      pool_variant<bexk::RolePairLayout64<true>>("rolepair64 x2 waves, M in registers"),
       pool_variant<bexk::GroupLayout<64, 1, true>>("group64 x1 wave"),
       pool_variant<bexk::GroupLayout<64, 2, true>>("group64 x2 waves")};
   static const std::vector<PoolVariant> v128 = {

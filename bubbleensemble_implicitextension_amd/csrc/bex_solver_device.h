@@ -1281,6 +1281,184 @@ struct ChainLayout128 {
   }
 };
 
+// =========================================================================================
+// RolePairLayout64: one N = 64 system per workgroup of two waves, the coupling matrix in
+// registers. Wave B serves units 32 B .. 32 B + 31 as a role pair (primal half: row u of M and
+// r, p, q, x; shadow half: row u of M^T and r~, p~, q~), each lane holding its row's 64
+// entries (128 VGPRs) and running all four FMA chains of its row product like
+// RolePairLayout32. A product's operand passes through LDS once, so that lane k of each half
+// holds z_k and z_(32 + k) for the DPP row_newbcast broadcasts (two LDS buffers used in turn:
+// the barrier of one product orders the next product's stores after the previous reads);
+// reductions over the 64 units take their first halving level (h = 32, across the two waves)
+// from LDS and the rest in-wave.
+template <bool PS = true>
+struct RolePairLayout64 {
+  static constexpr int N = 64;
+  static constexpr int kUnits = N;
+  static constexpr int kWaves = 2;
+  static constexpr int kBlock = kWaves * kWave;
+  static constexpr int kSystemsPerBlock = 1;
+  static constexpr int kWavesPerEU = 2;  // 128 VGPRs of matrix per lane: 256-register budget
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 1;
+
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * (2 * 2 * N + N + N) + sizeof(double2) * N + sizeof(Sys) +
+           sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
+  }
+
+  int tid, hh, k, blk, u, sid_;
+  mutable int zsel;       // LDS operand buffer of the next product
+  double mrow[64];        // row u of M (primal) or of M^T (shadow)
+  double* zbuf;           // [2 buffers][2 halves][32][2]: (z_k, z_(32+k))
+  double* red;            // [32][2 blocks]
+  double* redm;           // [32][2 blocks]
+  double2* red2;          // [32][2 blocks]
+  Sys* sys;
+  int* stat;
+  int* flags;
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs&) {
+    tid = threadIdx.x;
+    const int l = tid & (kWave - 1);
+    blk = tid / kWave;
+    hh = l >> 5;
+    k = l & 31;
+    u = 32 * blk + k;
+    zsel = 0;
+    double* p = reinterpret_cast<double*>(smem);
+    zbuf = p;
+    red = zbuf + 2 * 2 * N;
+    redm = red + N;
+    red2 = reinterpret_cast<double2*>(redm + N);
+    sys = reinterpret_cast<Sys*>(red2 + N);
+    stat = reinterpret_cast<int*>(sys + 1);
+    flags = stat + BEX_NSTAT;
+  }
+
+  // Work item = one system: loads element (u, j) of M (primal) or M^T (shadow), j = 0..63; a
+  // per-system matrix has no stored transpose, so the shadow half reads column u of M.
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    int s = -1;
+    if (item < count) s = order ? order[item] : a.first_system + (int)item;
+    sid_ = s;
+    if (tid < BEX_NSTAT) stat[tid] = 0;
+    if (s >= 0 && s < a.NS) {
+      const double* m;
+      size_t su, sj;  // strides of u and j
+      if (PS) {
+        m = a.CouplingMatrix + (size_t)s * N * N;
+        su = hh == 0 ? 1 : N;
+        sj = hh == 0 ? N : 1;
+      } else {
+        m = hh == 0 ? a.CouplingMatrix : a.CouplingMatrixT;
+        su = 1;
+        sj = N;
+      }
+      const double* src = m + (size_t)u * su;
+#pragma unroll
+      for (int j = 0; j < 64; j++) mrow[j] = src[(size_t)j * sj];
+    }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return u; }
+  __device__ __forceinline__ bool primal() const { return hh == 0; }
+  __device__ __forceinline__ bool primal_half() const { return hh == 0; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+
+  __device__ __forceinline__ bool all(bool p) const {
+    if (tid == 0) flags[0] = 0;
+    __syncthreads();
+    if (!p) flags[0] = 1;
+    __syncthreads();
+    const bool r = flags[0] == 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool any(bool p) const {
+    if (tid == 0) flags[1] = 0;
+    __syncthreads();
+    if (p) flags[1] = 1;
+    __syncthreads();
+    const bool r = flags[1] != 0;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ bool uniform_any(bool p) const { return __ballot(p) != 0ull; }
+
+  // Minimum over the units (positive, non-NaN operands: see RolePairLayout32::min_keep).
+  __device__ __forceinline__ double min_keep(double v) const {
+    if (primal()) redm[k * 2 + blk] = v;
+    __syncthreads();
+    const double r = uniform(tree32(VminOp()(redm[k * 2], redm[k * 2 + 1]), VminOp()));
+    __syncthreads();
+    return r;
+  }
+
+  // M z (primal half) and M^T z (shadow half) for the per-unit operand z, canonical order.
+  __device__ __forceinline__ double matvec(double z) const {
+    double* zb = zbuf + zsel * (2 * N);
+    zsel ^= 1;
+    zb[(hh * 32 + k) * 2 + blk] = z;
+    __syncthreads();
+    const double2 zz = reinterpret_cast<const double2*>(zb)[hh * 32 + k];
+    double e0, o0, e1, o1;
+    rows_of_half(zz.x, &e0, &o0);  // z_c, z_(16+c)
+    rows_of_half(zz.y, &e1, &o1);  // z_(32+c), z_(48+c)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    fmac_row16(acc, e0, &mrow[0]);
+    fmac_row16(acc, o0, &mrow[16]);
+    fmac_row16(acc, e1, &mrow[32]);
+    fmac_row16(acc, o1, &mrow[48]);
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+
+  // Sum over the 64 units of primal(v) * shadow(v): level h = 32 across the two waves in LDS,
+  // then the in-wave tree; on every lane.
+  __device__ __forceinline__ double dot_halves(double v) const {
+    double lo, up;
+    halves(v, &lo, &up);
+    if (primal()) red[k * 2 + blk] = lo * up;
+    __syncthreads();
+    const double2 v01 = reinterpret_cast<const double2*>(red)[k];
+    return tree32(v01.x + v01.y, AddOp());
+  }
+
+  // max |r| and r . r~ over the 64 units, on every lane.
+  __device__ __forceinline__ void res_trees(double res, double* rmax, double* rho) const {
+    double lo2, up2;
+    halves(res, &lo2, &up2);
+    if (primal()) red2[k * 2 + blk] = make_double2(bex::fabs_d(lo2), lo2 * up2);
+    __syncthreads();
+    const double2 w0 = red2[k * 2], w1 = red2[k * 2 + 1];
+    double m = VmaxOp()(w0.x, w1.x);
+    double p = w0.y + w1.y;
+    tree32x2(m, p, VmaxOp(), AddOp());
+    *rmax = m;
+    *rho = p;
+  }
+
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    const double b = fma(rD, matvec(cpt), xu);  // meaningful on the primal half
+    const double r0 = rp_residual(*this, b, xu, x1sq, rD);
+    double x;
+    const BicgOut out = rp_bicg(*this, r0, xu, x1sq, rD, a, &x);
+    double lo, up;
+    halves(x, &lo, &up);
+    F[1] = lo;
+    return out;
+  }
+};
+
 // Adds v to the system's counter k in LDS. An LDS atomic whose result is unused needs no wait,
 // where a read-modify-write would stall the wave for the LDS round trip.
 __device__ __forceinline__ void count(int* st, int k, int v) { atomicAdd(&st[k], v); }

@@ -95,10 +95,11 @@ enum bex_option {
   BEX_DenseOutputInterpolation
 };
 
-#define BEX_LAYOUT_AUTO 0      /* role pairs for 32 and 128 units, lane groups otherwise */
+#define BEX_LAYOUT_AUTO 0      /* role pairs for 32, 64 and 128 units, lane groups otherwise */
 #define BEX_LAYOUT_GROUP 1     /* lane = unit, 64/N systems per wave, M in LDS */
 #define BEX_LAYOUT_ROLE_PAIR 2 /* 32 units: one system per wave, M / M^T rows in registers;
-                                  128 units: eight waves per system, M / M^T in registers */
+                                  64 and 128 units: two and eight waves per system, M / M^T in
+                                  registers */
 
 /* The reference's template parameters (Interface.cuh:135). */
 typedef struct {

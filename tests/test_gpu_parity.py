@@ -3,7 +3,8 @@
 The canonical oracle restates the reference solver path with the same elementary functions
 and association order as the kernels, so every state value, accessory, event counter and
 step/iteration counter must agree exactly, including on chaotic systems. Both lane layouts of
-the 32-unit kernel (role pairs with DPP broadcasts, lane groups with LDS) are checked.
+the 32-, 64- and 128-unit kernels (role pairs with the matrix in registers and DPP broadcasts,
+lane groups with the matrix in LDS) are checked.
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
 pytestmark = pytest.mark.gpu
 
 CASES = [(8, LAYOUT_GROUP), (16, LAYOUT_GROUP), (32, LAYOUT_GROUP), (32, LAYOUT_ROLE_PAIR),
-         (64, LAYOUT_GROUP)]
+         (64, LAYOUT_GROUP), (64, LAYOUT_ROLE_PAIR)]
 
 
 @pytest.mark.parametrize("N,layout", CASES)
