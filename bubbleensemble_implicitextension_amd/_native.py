@@ -96,6 +96,8 @@ _SIGNATURES = {
     "bex_pool_set_order": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
                            ctypes.c_int),
     "bex_pool_order_by_cost": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bex_pool_get_order": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                           ctypes.c_int),
     "bex_pool_solve": ([ctypes.c_void_p], ctypes.c_int),
     "bex_pool_solve_periods": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "bex_pool_synchronise": ([ctypes.c_void_p], ctypes.c_int),

@@ -1398,6 +1398,15 @@ int bex_pool_set_order(bex_pool* p, int c, const int* order) {
   return BEX_OK;
 }
 
+int bex_pool_get_order(bex_pool* p, int c, int* order) {
+  if (!p || !order || c < 0 || c >= (int)p->cls.size()) return BEX_E_ARGUMENT;
+  bex_pool_class& k = p->cls[c];
+  BEX_PHIP(p, hipSetDevice(p->device));
+  BEX_PHIP(p, hipStreamSynchronize(p->stream));
+  BEX_PHIP(p, hipMemcpy(order, k.d_order, sizeof(int) * (size_t)k.NS, hipMemcpyDeviceToHost));
+  return BEX_OK;
+}
+
 // Orders class c's systems by the BiCG iterations they needed so far (their accumulated
 // counters), most expensive first; ties keep the index order.
 int bex_pool_order_by_cost(bex_pool* p, int c) {
@@ -1530,6 +1539,25 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
       }
     }
     slot += nv;
+    // Every probe did the same per-system work (results do not depend on the variant): order
+    // the class's work queue by it, longest first (LPT), as bex_pool_order_by_cost does with
+    // the counters of earlier periods.
+    std::vector<int> order;
+    if (rc == BEX_OK && stats_saved) {
+      std::vector<long long> now((size_t)k.NS * BEX_NSTAT), before((size_t)k.NS * BEX_NSTAT);
+      if (check(hipMemcpy(now.data(), s->d_stats, sbytes, hipMemcpyDeviceToHost), "hipMemcpy") &&
+          check(hipMemcpy(before.data(), stats_saved, sbytes, hipMemcpyDeviceToHost), "hipMemcpy")) {
+        std::vector<long long> work(k.NS);
+        for (int i = 0; i < k.NS; i++) {
+          const size_t b = (size_t)i * BEX_NSTAT;
+          work[i] = (now[b + BEX_STAT_BICG_ITERATIONS] + now[b + BEX_STAT_STAGES]) -
+                    (before[b + BEX_STAT_BICG_ITERATIONS] + before[b + BEX_STAT_STAGES]);
+        }
+        order.resize(k.NS);
+        for (int i = 0; i < k.NS; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+      }
+    }
     // Restore the snapshot whatever happened, then release it.
     for (size_t b = 0; b < nkeep; b++)
       if (saved[b]) check(hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice),
@@ -1544,6 +1572,10 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
     if (rc != BEX_OK) {
       p->graph_valid = false;
       return pfail(p, rc, "bex_pool_autotune: " + msg);
+    }
+    if (!order.empty()) {
+      const int orc = bex_pool_set_order(p, (int)c, order.data());
+      if (orc != BEX_OK) return orc;
     }
   }
   p->graph_valid = false;

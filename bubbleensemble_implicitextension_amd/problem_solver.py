@@ -354,6 +354,13 @@ class ProblemPool:
     def order_by_cost(self, c):
         self._check(self._lib.bex_pool_order_by_cost(self._h, c))
 
+    def get_order(self, c):
+        """The order in which class c's systems are handed out."""
+        o = np.zeros(self.systems[c], dtype=np.int32)
+        self._check(self._lib.bex_pool_get_order(self._h, c,
+                                                 o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return o
+
     def Solve(self):
         self._check(self._lib.bex_pool_solve(self._h))
 
@@ -377,7 +384,8 @@ class ProblemPool:
         return v.value
 
     def autotune(self, probe_fraction=0.05):
-        """Returns {class: {variant name: probe ms}} and leaves the fastest variant selected."""
+        """Returns {class: {variant name: probe ms}}, leaves the fastest variant selected and
+        each class's work queue ordered by the probe's per-system work, longest first."""
         counts = [self._lib.bex_pool_variant_count(n) for n in self.units]
         times = (ctypes.c_double * sum(counts))()
         self._check(self._lib.bex_pool_autotune(self._h, float(probe_fraction), times))

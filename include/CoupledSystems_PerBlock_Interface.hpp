@@ -279,6 +279,7 @@ class ProblemSolver {
 class ProblemPool {
  private:
   bex_pool* pool_ = nullptr;
+  std::vector<int> ns_;  // systems per class
 
   void Check(int rc, const char* function) const {
     if (rc == BEX_OK) return;
@@ -293,6 +294,7 @@ class ProblemPool {
       std::cerr << "ERROR: ProblemPool: units and systems differ in length" << std::endl;
       std::exit(EXIT_FAILURE);
     }
+    ns_ = systems;
     if (bex_pool_create((int)units.size(), units.data(), systems.data(), device, &pool_) !=
         BEX_OK) {
       std::cerr << "ERROR: creating the pool failed!" << std::endl;
@@ -318,6 +320,11 @@ class ProblemPool {
     Check(bex_pool_set_order(pool_, c, order.data()), "SetOrder");
   }
   void OrderByCost(int c) { Check(bex_pool_order_by_cost(pool_, c), "OrderByCost"); }
+  std::vector<int> GetOrder(int c) {
+    std::vector<int> order((size_t)ns_.at(c));
+    Check(bex_pool_get_order(pool_, c, order.data()), "GetOrder");
+    return order;
+  }
   void Autotune(double probe_fraction) {
     Check(bex_pool_autotune(pool_, probe_fraction, nullptr), "Autotune");
   }

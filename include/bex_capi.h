@@ -224,6 +224,8 @@ int bex_pool_set_couplings(bex_pool* p, int c, int first, int count, const doubl
 int bex_pool_set_order(bex_pool* p, int c, const int* order);
 /* Orders class c by the BiCG work its systems have accumulated (counters), largest first. */
 int bex_pool_order_by_cost(bex_pool* p, int c);
+/* The order in which class c's systems are handed out (NS entries). */
+int bex_pool_get_order(bex_pool* p, int c, int* order);
 int bex_pool_solve(bex_pool* p);
 int bex_pool_solve_periods(bex_pool* p, int periods);
 int bex_pool_synchronise(bex_pool* p);
@@ -234,7 +236,9 @@ int bex_pool_set_variant(bex_pool* p, int c, int variant);
 int bex_pool_get_variant(bex_pool* p, int c, int* variant);
 /* Autotuner: times every variant of every class on the first probe_fraction of the current
  * time domain (state, counters and time domain restored afterwards) and selects the fastest.
- * times_ms (optional) receives the times, class by class, variant by variant. */
+ * It also orders each class's work queue by the per-system work of the probe, longest first
+ * (call bex_pool_set_order afterwards to override). times_ms (optional) receives the times,
+ * class by class, variant by variant. No reference counterpart. */
 int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms);
 const char* bex_pool_last_error(const bex_pool* p);
 

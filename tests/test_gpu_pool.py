@@ -140,7 +140,18 @@ def test_pool_autotune_restores_state():
         s.SynchroniseSolver()
         assert_bitwise(f"class {c} state after autotune", s.host_view(V.ActualState), before[c])
         assert not s.stats().any()
+    # The autotuner leaves the queues in longest-probe-first order: the same results.
+    pool.Solve()
+    tuned = _results(pool, classes)
     pool.close()
+    print("case: pool without autotune", flush=True)
+    ref = _pool(classes)
+    ref.Solve()
+    plain = _results(ref, classes)
+    ref.close()
+    for c in range(len(classes)):
+        for key in tuned[c]:
+            assert_bitwise(f"class {c} {key} (autotuned)", tuned[c][key], plain[c][key])
 
 
 def test_config4_class_matches_oracle(oracle):
