@@ -1459,6 +1459,33 @@ struct RolePairLayout64 {
   }
 };
 
+// Cash-Karp tableau (Steppers.cuh:197-370): row s holds a_s1..a_s5 and c_s. Read through
+// rkck45_row(s, dep): the row's address passes through an empty asm tied to dep, so the scalar
+// loads happen at the stage (not hoisted out of the time loop into scalar registers, where the
+// ~20 constants would compete with the solver's uniform values).
+static __constant__ const double kRkck45[6][8] = {
+    {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {1.0 / 5.0, 0.0, 0.0, 0.0, 0.0, 1.0 / 5.0, 0.0, 0.0},
+    {3.0 / 40.0, 9.0 / 40.0, 0.0, 0.0, 0.0, 3.0 / 10.0, 0.0, 0.0},
+    {3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0, 0.0, 0.0, 3.0 / 5.0, 0.0, 0.0},
+    {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0, 0.0, 1.0, 0.0, 0.0},
+    {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0,
+     7.0 / 8.0, 0.0, 0.0}};
+
+typedef const __attribute__((address_space(4))) double* ConstRow;
+
+__device__ __forceinline__ ConstRow rkck45_row(int s, double dep) {
+  ConstRow p = (ConstRow)&kRkck45[0][0];
+  asm("" : "+s"(p) : "v"(dep));
+  return p + 8 * s;
+}
+
+template <int UD>
+__device__ __forceinline__ void store_stage(double (&k)[UD], const double* F) {
+#pragma unroll
+  for (int i = 0; i < UD; i++) k[i] = F[i];
+}
+
 // Adds v to the system's counter k in LDS. An LDS atomic whose result is unused needs no wait,
 // where a read-modify-write would stall the wave for the LDS round trip.
 __device__ __forceinline__ void count(int* st, int k, int v) { atomicAdd(&st[k], v); }
@@ -1647,29 +1674,50 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     for (int s = 0; s < nstage; s++) {
       double xs[UD], F[UD];
       double Ts;
+      if (ALG == kRKCK45) {
+        // One branch per stage for both components and the stage time; the tableau row is read
+        // from constant memory with scalar loads.
+        const ConstRow A = rkck45_row(s, t);
+        switch (s) {
+          case 0:
 #pragma unroll
-      for (int i = 0; i < UD; i++) {
-        if (ALG == kRKCK45) {
-          switch (s) {
-            case 0: xs[i] = x[i]; break;
-            case 1: xs[i] = x[i] + (k1[i] * (1.0 / 5.0)) * dt; break;
-            case 2: xs[i] = x[i] + (k1[i] * (3.0 / 40.0) + k2[i] * (9.0 / 40.0)) * dt; break;
-            case 3:
-              xs[i] = x[i] + (k1[i] * (3.0 / 10.0) + k2[i] * (-9.0 / 10.0) + k3[i] * (6.0 / 5.0)) * dt;
-              break;
-            case 4:
-              xs[i] = x[i] + (k1[i] * (-11.0 / 54.0) + k2[i] * (5.0 / 2.0) +
-                              k3[i] * (-70.0 / 27.0) + k4[i] * (35.0 / 27.0)) *
+            for (int i = 0; i < UD; i++) xs[i] = x[i];
+            Ts = t;
+            break;
+          case 1:
+#pragma unroll
+            for (int i = 0; i < UD; i++) xs[i] = x[i] + (k1[i] * A[0]) * dt;
+            Ts = t + A[5] * dt;
+            break;
+          case 2:
+#pragma unroll
+            for (int i = 0; i < UD; i++) xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1]) * dt;
+            Ts = t + A[5] * dt;
+            break;
+          case 3:
+#pragma unroll
+            for (int i = 0; i < UD; i++)
+              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2]) * dt;
+            Ts = t + A[5] * dt;
+            break;
+          case 4:
+#pragma unroll
+            for (int i = 0; i < UD; i++)
+              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2] + k4[i] * A[3]) * dt;
+            Ts = t + dt;
+            break;
+          default:
+#pragma unroll
+            for (int i = 0; i < UD; i++)
+              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2] + k4[i] * A[3] +
+                              k5[i] * A[4]) *
                                  dt;
-              break;
-            default:
-              xs[i] = x[i] + (k1[i] * (1631.0 / 55296.0) + k2[i] * (175.0 / 512.0) +
-                              k3[i] * (575.0 / 13824.0) + k4[i] * (44275.0 / 110592.0) +
-                              k5[i] * (253.0 / 4096.0)) *
-                                 dt;
-              break;
-          }
-        } else {
+            Ts = t + A[5] * dt;
+            break;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < UD; i++) {
           switch (s) {
             case 0: xs[i] = x[i]; break;
             case 1: xs[i] = x[i] + k1[i] * (0.5 * dt); break;
@@ -1677,17 +1725,6 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
             default: xs[i] = x[i] + k3[i] * dt; break;
           }
         }
-      }
-      if (ALG == kRKCK45) {
-        switch (s) {
-          case 0: Ts = t; break;
-          case 1: Ts = t + (1.0 / 5.0) * dt; break;
-          case 2: Ts = t + (3.0 / 10.0) * dt; break;
-          case 3: Ts = t + (3.0 / 5.0) * dt; break;
-          case 4: Ts = t + dt; break;
-          default: Ts = t + (7.0 / 8.0) * dt; break;
-        }
-      } else {
         Ts = (s == 0) ? t : ((s == 3) ? t + dt : t + 0.5 * dt);
       }
       // Right-hand side and coupling solve (Steppers.cuh:431-562).
@@ -1714,16 +1751,13 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         count(st, BEX_STAT_BICG_MAXIT, o.maxit ? 1 : 0);
         count(st, BEX_STAT_HIST0 + (o.iterations < 20 ? o.iterations : 19), 1);
       }
-#pragma unroll
-      for (int i = 0; i < UD; i++) {
-        switch (s) {
-          case 0: k1[i] = F[i]; break;
-          case 1: k2[i] = F[i]; break;
-          case 2: k3[i] = F[i]; break;
-          case 3: k4[i] = F[i]; break;
-          case 4: k5[i] = F[i]; break;
-          default: k6[i] = F[i]; break;
-        }
+      switch (s) {
+        case 0: store_stage(k1, F); break;
+        case 1: store_stage(k2, F); break;
+        case 2: store_stage(k3, F); break;
+        case 3: store_stage(k4, F); break;
+        case 4: store_stage(k5, F); break;
+        default: store_stage(k6, F); break;
       }
     }
     // k1 = f(t, x) is the pending step's end-point derivative.
