@@ -772,7 +772,8 @@ __device__ __forceinline__ float mlp_factor(const BexMlpDevice* net, const doubl
 // units and the shadow half r~, p~, q~ in the same registers (one instruction stream serves
 // A p and A^T p~). The layout L provides matvec(z) (M z on the primal half, M^T z on the shadow
 // half, for a per-unit z), dot_halves(v) (sum over the units of primal(v) * shadow(v), on
-// every lane), res_trees(r, &max|r|, &r.r~), uniform_any(p) and primal_half().
+// every lane), res_test(r, tol, &r.r~) (whether max |r| < tol, with the reference's NaN-ignoring
+// max), res_max(r), uniform_any(p) and primal_half().
 
 // Initial residual r0 = b - A x0 of the primal half, on both halves.
 template <class L>
@@ -813,15 +814,15 @@ __device__ __forceinline__ BicgOut rp_bicg(const L& lay, double res0, double x0,
         x = x + alpha * dir;
         res = res - alpha * qv;
         const double rho2 = rho1;
-        // The residual test and the next r . r~ are independent: reduced together.
-        double rmax;
-        lay.res_trees(res, &rmax, &rho1);
-        out.rmax = rmax;
-        if (lay.uniform_any(rmax < a.BiCGResidualTolerance)) {
+        // The residual test (max |r| < tolerance) and the next r . r~.
+        if (lay.res_test(res, a.BiCGResidualTolerance, &rho1)) {
           out.maxit = false;
           break;
         }
-        if (iter >= maxit) break;
+        if (iter >= maxit) {
+          out.rmax = lay.res_max(res);  // consulted by the learned start's restart rule
+          break;
+        }
         const double beta = rho1 / rho2;
         dir = res + beta * dir;
       }
@@ -935,20 +936,40 @@ struct RolePairLayout32 {
     return tree32(lo * up, AddOp());
   }
 
-  // The residual test's max |r| over the primal half and r . r~, on every lane of both halves.
-  // Their first level (h = 16) is formed from the two rows of r and r~: max(|r_c|, |r_c+16|) and
-  // r_c r~_c + r_c+16 r~_c+16, the same operations as on |r| and the products.
-  __device__ __forceinline__ void res_trees(double res, double* rmax, double* rho) const {
+  // r . r~ on every lane (its first level, h = 16, from the two rows of r and r~:
+  // r_c r~_c + r_c+16 r~_c+16) and the residual test max |r| < tol over the primal half. With
+  // the reference's NaN-ignoring max the test holds exactly when no non-NaN |r_i| fails
+  // |r_i| < tol and some r_i is not NaN: two ballots instead of a max tree.
+  __device__ __forceinline__ bool res_test(double res, double tol, double* rho) const {
     double lo2, up2;
     halves(res, &lo2, &up2);  // r and r~ of the lane's unit, on both halves
     double re, ro, te, to;
     rows_of_half(lo2, &re, &ro);
     rows_of_half(up2, &te, &to);
-    double m = VmaxAbsOp()(re, ro);
     double p = re * te + ro * to;
-    tree16x2(m, p, VmaxOp(), AddOp());
-    *rmax = m;
+    p = AddOp()(p, row_ror<8>(p));
+    p = AddOp()(p, row_ror<4>(p));
+    p = AddOp()(p, row_ror<2>(p));
+    p = AddOp()(p, row_ror<1>(p));
     *rho = p;
+    // Lane masks straight from the compares (11: unordered or >=, 7: ordered); the primal half
+    // is the low 32 lanes.
+    const unsigned long long numbers = __builtin_amdgcn_fcmp(res, res, 7);
+    const unsigned long long failing = __builtin_amdgcn_fcmp(bex::fabs_d(res), tol, 11) & numbers;
+    return (unsigned)failing == 0u && (unsigned)numbers != 0u;
+  }
+  // max |r| over the primal half (NaN ignored), on every lane.
+  __device__ __forceinline__ double res_max(double res) const {
+    double lo2, up2;
+    halves(res, &lo2, &up2);
+    double re, ro;
+    rows_of_half(lo2, &re, &ro);
+    double m = VmaxAbsOp()(re, ro);
+    m = VmaxOp()(m, row_ror<8>(m));
+    m = VmaxOp()(m, row_ror<4>(m));
+    m = VmaxOp()(m, row_ror<2>(m));
+    m = VmaxOp()(m, row_ror<1>(m));
+    return m;
   }
   // Branch on a predicate that has the same value in every lane.
   __device__ __forceinline__ bool uniform_any(bool p) const { return any(p); }
@@ -1264,6 +1285,16 @@ struct ChainLayout128 {
     *rmax = m;
     *rho = p;
   }
+  __device__ __forceinline__ bool res_test(double res, double tol, double* rho) const {
+    double m;
+    res_trees(res, &m, rho);
+    return uniform_any(m < tol);
+  }
+  __device__ __forceinline__ double res_max(double res) const {
+    double m, p;
+    res_trees(res, &m, &p);
+    return m;
+  }
 
   template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
@@ -1441,6 +1472,16 @@ struct RolePairLayout64 {
     tree32x2(m, p, VmaxOp(), AddOp());
     *rmax = m;
     *rho = p;
+  }
+  __device__ __forceinline__ bool res_test(double res, double tol, double* rho) const {
+    double m;
+    res_trees(res, &m, rho);
+    return uniform_any(m < tol);
+  }
+  __device__ __forceinline__ double res_max(double res) const {
+    double m, p;
+    res_trees(res, &m, &p);
+    return m;
   }
 
   template <class Feat>
