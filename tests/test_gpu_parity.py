@@ -43,12 +43,15 @@ def test_wide_layouts_bitexact(oracle, N, end, layout):
     s.close()
 
 
-@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
-def test_rk4_bitexact(oracle, layout):
-    ens = wl.small_ensemble(n=32, ns=4, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))
+@pytest.mark.parametrize("N,layout", [(32, LAYOUT_GROUP), (32, LAYOUT_ROLE_PAIR),
+                                      (64, LAYOUT_ROLE_PAIR), (128, LAYOUT_ROLE_PAIR)])
+def test_rk4_bitexact(oracle, N, layout):
+    ens = wl.small_ensemble(n=N, ns=4 if N == 32 else 2, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))
     opts = {"InitialTimeStep": 2e-4}
-    s = run_gpu(ens, periods=1, algorithm=0, options=opts, layout=layout)
-    p = run_oracle(oracle, ens, periods=1, algorithm=0, options=opts)
+    end = 1.0 if N == 32 else 0.2
+    print(f"rk4 N={N} layout={layout} end={end}", flush=True)
+    s = run_gpu(ens, periods=1, algorithm=0, options=opts, layout=layout, end_time=end)
+    p = run_oracle(oracle, ens, periods=1, algorithm=0, options=opts, end_time=end)
     compare_solver_to_oracle(s, p)
 
 
@@ -97,19 +100,23 @@ def test_config0_within_config1_batch(oracle):
     np.testing.assert_array_equal(s.stats()[sids[keep]], p.stats_table())
 
 
-@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
+@pytest.mark.parametrize("N,layout,periods,end", [(32, LAYOUT_GROUP, 2, 1.0),
+                                                   (32, LAYOUT_ROLE_PAIR, 2, 1.0),
+                                                   (64, LAYOUT_ROLE_PAIR, 2, 0.3),
+                                                   (128, LAYOUT_ROLE_PAIR, 2, 0.1)])
 @pytest.mark.parametrize("interp", [0, 1])
-def test_dense_output_bitexact(oracle, layout, interp):
-    """Dense output, step-point sampling (reference) and Hermite grid, two periods so that the
-    persistent index carries across Solve() calls."""
+def test_dense_output_bitexact(oracle, N, layout, periods, end, interp):
+    """Dense output, step-point sampling (reference) and Hermite grid, two Solve() calls so
+    that the persistent index carries across them; the 64- and 128-unit systems run the
+    register-resident multi-wave role-pair layouts on a shortened time domain."""
     from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
-    ens = wl.small_ensemble(n=32, ns=4, seed=9)
+    ens = wl.small_ensemble(n=N, ns=4 if N == 32 else 2, seed=9 + N)
     ndo = 100
-    opts = {"DenseOutputMinimumTimeStep": 1.0 / 40.0, "DenseOutputInterpolation": interp}
-    print(f"dense output N=32 layout={layout} interp={interp}", flush=True)
-    s = run_gpu(ens, periods=2, options=opts, ndo=ndo, layout=layout)
-    p = run_oracle(oracle, ens, periods=2, options=opts, ndo=ndo)
+    opts = {"DenseOutputMinimumTimeStep": end / 40.0, "DenseOutputInterpolation": interp}
+    print(f"dense output N={N} layout={layout} interp={interp} end={end}", flush=True)
+    s = run_gpu(ens, periods=periods, options=opts, ndo=ndo, layout=layout, end_time=end)
+    p = run_oracle(oracle, ens, periods=periods, options=opts, ndo=ndo, end_time=end)
     compare_solver_to_oracle(s, p)
     from gpu_helpers import assert_bitwise
     assert_bitwise("DenseIndex", s.host_view(V.DenseIndex), p.DenseOutputIndex)
