@@ -120,22 +120,38 @@ __forceinline__ __device__ __host__ void CoupledSystems_PerBlock_Initialization(
 // solver. Each feature is scaled to O(1) over the solver's operating range:
 //   WF[0] = log(R/R_E), WF[1] = 10 Rdot/c_L, WF[2] = forcing pressure / C0 (both frequencies),
 //   WF[3] = log(1 + CRS), WF[4] = log(dT) / 10.
+// The network reads its inputs in bfloat16 (8 significant bits), so the features are evaluated
+// in single precision with the hardware's log2 and sine (the sine's argument in revolutions);
+// the solution itself never depends on them beyond the starting guess.
+namespace bexfeat {
+__forceinline__ __device__ __host__ float log_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+#else
+  return logf(x);
+#endif
+}
+// sin(2 pi x)
+__forceinline__ __device__ __host__ float sin_rev(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sinf(x - rintf(x));
+#else
+  return sinf(6.283185307179586f * (x - rintf(x)));
+#endif
+}
+}  // namespace bexfeat
+
 template <class Precision>
 __forceinline__ __device__ __host__ void CoupledSystems_PerBlock_WarmStartFeatures(
     int sid, int uid, Precision* WF, Precision T, Precision dT, Precision* X, Precision* uPAR,
     Precision* sPAR, Precision* gPAR, int* igPAR, Precision CRS) {
-  Precision lo;
-  Precision s1;
-  Precision c1;
-  bex::sincospi(2.0 * T, &s1, &c1);
-  Precision s2;
-  Precision c2;
-  bex::sincos(2.0 * uPAR[11] * PI * T + uPAR[12], &s2, &c2);
-  WF[0] = bex::log_dd(X[0], &lo);
+  const float s1 = bexfeat::sin_rev((float)T);
+  const float s2 = bexfeat::sin_rev((float)(uPAR[11] * T + uPAR[12] * (0.5 / PI)));
+  WF[0] = bexfeat::log_f((float)X[0]);
   WF[1] = 10.0 * uPAR[9] * X[1];
-  WF[2] = (uPAR[5] * s1 + uPAR[6] * s2) / uPAR[0];
-  WF[3] = bex::log_dd(1.0 + CRS, &lo);
-  WF[4] = 0.1 * bex::log_dd(dT, &lo);
+  WF[2] = ((float)uPAR[5] * s1 + (float)uPAR[6] * s2) / (float)uPAR[0];
+  WF[3] = bexfeat::log_f((float)(1.0 + CRS));
+  WF[4] = 0.1f * bexfeat::log_f((float)dT);
 }
 
 // BubbleEnsemble_SystemDefinition.cuh:118-126
