@@ -1664,7 +1664,8 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       const int idx = S.DOIDX;
       if (idx >= a.NDO) break;
       const double tA = S.tA, tB = S.tB;
-      const double tq = S.gridT0 + (double)S.kq * a.DenseOutputMinimumTimeStep;
+      const int kq = S.kq;  // read by every thread before the barrier, like idx
+      const double tq = S.gridT0 + (double)kq * a.DenseOutputMinimumTimeStep;
       if (!(tq <= tB)) break;
       const double h = tB - tA;
       const double th = (tq - tA) / h;
@@ -1684,9 +1685,11 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         if (uid == 0) a.DenseOutputTimeInstances[sid + (long long)idx * NS] = tq;
       }
       lay.sync();
+      // Unit 0 may be served by threads of several waves: they store values read before the
+      // barrier, never a read-modify-write of the record.
       if (uid == 0) {
         S.DOIDX = idx + 1;
-        S.kq = S.kq + 1;
+        S.kq = kq + 1;
       }
       lay.sync();
     }
@@ -1804,6 +1807,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     // k1 = f(t, x) is the pending step's end-point derivative.
     if (interp && S.pending != 0) {
       dense_flush(k1);
+      lay.sync();  // every wave has read the flag (an empty flush has no barrier of its own)
       if (uid == 0) S.pending = 0;
       lay.sync();
     }
