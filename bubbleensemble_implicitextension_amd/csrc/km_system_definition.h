@@ -49,9 +49,12 @@ __forceinline__ __device__ __host__ void CoupledSystems_PerBlock_OdeFunction(
   Precision c1;
   bex::sincospi(2.0 * T, &s1, &c1);
 
-  Precision s2;
-  Precision c2;
-  bex::sincos(2.0 * uPAR[11] * PI * T + uPAR[12], &s2, &c2);
+  // Second forcing frequency. bex::sincos(+-0) is (+0, 1) exactly, so the call is skipped for a
+  // zero argument (f2 = theta = 0, single-frequency forcing) with the same result.
+  Precision s2 = 0.0;
+  Precision c2 = 1.0;
+  const Precision a2 = 2.0 * uPAR[11] * PI * T + uPAR[12];
+  if (a2 != 0.0) bex::sincos(a2, &s2, &c2);
 
   Precision N = (uPAR[0] + uPAR[1] * X[1]) * p - uPAR[2] * (1.0 + uPAR[9] * X[1]) -
                 uPAR[3] * rx1 - uPAR[4] * X[1] * rx1 -

@@ -105,6 +105,11 @@ def test_detmath_trig_special_inputs(oracle):
             y = k * math.pi / 2 + d
             assert ulp_err(oracle.sin_(y), mp.sin(mp.mpf(y))) <= 2.0
             assert ulp_err(oracle.cos_(y), mp.cos(mp.mpf(y))) <= 2.0
+    # sin/cos of a zero argument of either sign are exactly +0 and 1: the right-hand side skips
+    # the second-frequency sincos for a zero argument on that basis.
+    for z in (0.0, -0.0):
+        s, c = oracle.sin_(z), oracle.cos_(z)
+        assert s == 0.0 and math.copysign(1.0, s) == 1.0 and c == 1.0
 
 
 def test_detmath_sincospi_accuracy(oracle):
