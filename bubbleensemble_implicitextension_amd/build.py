@@ -4,7 +4,8 @@
 
 The library is position independent, built for gfx950 only, with -ffp-contract=off so that the
 kernel's arithmetic is exactly the order written in the source (the CPU oracle uses the same
-order), and with packed FP32 instructions disabled.
+order), with packed FP32 instructions disabled, without the atomic optimizer and with loop heads
+aligned to 64 bytes.
 """
 from __future__ import annotations
 
@@ -28,6 +29,13 @@ COMMON_FLAGS = [
     "-ffp-contract=off",
     "-fno-gpu-rdc",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # The kernels' LDS counters are updated by one lane at a time: wave-level aggregation of
+    # those atomics (mbcnt/bcnt/exec juggling around every add) only adds instructions.
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
+    # Loop heads on 64-byte boundaries: a system's wave re-enters the BiCG loop about 10^6 times
+    # per period, and where its head falls in the instruction cache line shows up as ~3 % of
+    # that loop's time (measured on the tail system of the N=32 workload).
+    "-falign-loops=64",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
     "-I" + os.path.join(REPO, "include"),
     "-I" + CSRC,
