@@ -936,22 +936,12 @@ struct RolePairLayout32 {
     return tree32(lo * up, AddOp());
   }
 
-  // r . r~ on every lane (its first level, h = 16, from the two rows of r and r~:
-  // r_c r~_c + r_c+16 r~_c+16) and the residual test max |r| < tol over the primal half. With
-  // the reference's NaN-ignoring max the test holds exactly when no non-NaN |r_i| fails
-  // |r_i| < tol and some r_i is not NaN: two ballots instead of a max tree.
+  // r . r~ on every lane (the product of the two halves, then the tree, as p~ . q) and the
+  // residual test max |r| < tol over the primal half. With the reference's NaN-ignoring max the
+  // test holds exactly when no non-NaN |r_i| fails |r_i| < tol and some r_i is not NaN: two
+  // ballots instead of a max tree.
   __device__ __forceinline__ bool res_test(double res, double tol, double* rho) const {
-    double lo2, up2;
-    halves(res, &lo2, &up2);  // r and r~ of the lane's unit, on both halves
-    double re, ro, te, to;
-    rows_of_half(lo2, &re, &ro);
-    rows_of_half(up2, &te, &to);
-    double p = re * te + ro * to;
-    p = AddOp()(p, row_ror<8>(p));
-    p = AddOp()(p, row_ror<4>(p));
-    p = AddOp()(p, row_ror<2>(p));
-    p = AddOp()(p, row_ror<1>(p));
-    *rho = p;
+    *rho = dot_halves(res);
     // Lane masks straight from the compares (11: unordered or >=, 7: ordered); the primal half
     // is the low 32 lanes.
     const unsigned long long numbers = __builtin_amdgcn_fcmp(res, res, 7);
