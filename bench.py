@@ -229,7 +229,8 @@ def main():
         return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(systems are independent: shard them across GPUs by cost, see DESIGN.md)",
+              "(systems are independent: a multi-GPU run shards them by estimated cost, one "
+              "process and one solver per GPU)",
               file=sys.stderr)
         return 2
 
