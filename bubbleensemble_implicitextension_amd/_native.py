@@ -21,6 +21,21 @@ class ProblemDesc(ctypes.Structure):
                  "NUA", "NiUA", "NSA", "NiSA", "NE", "NDO", "Algorithm")]
 
 
+class DeviceProperties(ctypes.Structure):
+    """bex_device_properties (include/bex_capi.h)."""
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 256),
+                ("total_global_mem", ctypes.c_longlong),
+                ("shared_mem_per_block", ctypes.c_longlong),
+                ("total_const_mem", ctypes.c_longlong)] + \
+               [(n, ctypes.c_int) for n in
+                ("regs_per_block", "multiprocessor_count", "major", "minor", "clock_rate_khz",
+                 "memory_clock_rate_khz", "memory_bus_width", "warp_size",
+                 "max_threads_per_multiprocessor", "max_threads_per_block")] + \
+               [("max_threads_dim", ctypes.c_int * 3), ("max_grid_size", ctypes.c_int * 3)] + \
+               [(n, ctypes.c_int) for n in ("concurrent_copy", "concurrent_kernels",
+                                            "ecc_enabled")]
+
+
 _lib = None
 
 _SIGNATURES = {
@@ -110,6 +125,10 @@ _SIGNATURES = {
                           ctypes.c_int),
     "bex_pool_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
     "bex_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
+    "bex_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "bex_get_device_properties": ([ctypes.c_int, ctypes.POINTER(DeviceProperties)], ctypes.c_int),
+    "bex_select_device_by_closest_revision": ([ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

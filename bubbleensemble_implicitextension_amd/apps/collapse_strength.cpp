@@ -135,6 +135,9 @@ int main(int argc, char** argv) {
   std::vector<double> amp = grid(pamin, pamax, PARES, false);
   std::vector<double> freq = grid(frmin, frmax, FRRES, false);
 
+  ListCUDADevices();  // CollapseStrength.cu:80-81
+  PrintPropertiesOfSpecificDevice(DEVICE);
+
   Solver s(DEVICE);
   s.SolverOption(SharedCouplingMatrices, 0);
   s.SolverOption(InitialTimeStep, 1e-3);

@@ -367,6 +367,69 @@ const char* bex_last_error(const bex_solver* s) {
   return s ? s->err.c_str() : g_create_error.c_str();
 }
 
+int bex_device_count(int* count) {
+  if (!count) return fail(nullptr, BEX_E_ARGUMENT, "bex_device_count: null argument");
+  *count = 0;
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e == hipErrorNoDevice) return BEX_OK;
+  if (e != hipSuccess) return fail(nullptr, BEX_E_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  *count = n;
+  return BEX_OK;
+}
+
+int bex_get_device_properties(int device, bex_device_properties* out) {
+  if (!out) return fail(nullptr, BEX_E_ARGUMENT, "bex_get_device_properties: null argument");
+  int n = 0;
+  int rc = bex_device_count(&n);
+  if (rc != BEX_OK) return rc;
+  if (device < 0 || device >= n)
+    return fail(nullptr, BEX_E_BOUND, "device index " + std::to_string(device) + " out of range (" +
+                                          std::to_string(n) + " devices)");
+  hipDeviceProp_t p;
+  const hipError_t e = hipGetDeviceProperties(&p, device);
+  if (e != hipSuccess) return fail(nullptr, BEX_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+  std::memset(out, 0, sizeof(*out));
+  std::snprintf(out->name, sizeof out->name, "%s", p.name);
+  std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+  out->total_global_mem = (long long)p.totalGlobalMem;
+  out->shared_mem_per_block = (long long)p.sharedMemPerBlock;
+  out->total_const_mem = (long long)p.totalConstMem;
+  out->regs_per_block = p.regsPerBlock;
+  out->multiprocessor_count = p.multiProcessorCount;
+  out->major = p.major;
+  out->minor = p.minor;
+  out->clock_rate_khz = p.clockRate;
+  out->memory_clock_rate_khz = p.memoryClockRate;
+  out->memory_bus_width = p.memoryBusWidth;
+  out->warp_size = p.warpSize;
+  out->max_threads_per_multiprocessor = p.maxThreadsPerMultiProcessor;
+  out->max_threads_per_block = p.maxThreadsPerBlock;
+  for (int i = 0; i < 3; i++) {
+    out->max_threads_dim[i] = p.maxThreadsDim[i];
+    out->max_grid_size[i] = p.maxGridSize[i];
+  }
+  out->concurrent_copy = p.asyncEngineCount > 0 ? 1 : 0;
+  out->concurrent_kernels = p.concurrentKernels;
+  out->ecc_enabled = p.ECCEnabled;
+  return BEX_OK;
+}
+
+int bex_select_device_by_closest_revision(int major, int minor, int* device) {
+  if (!device) return fail(nullptr, BEX_E_ARGUMENT, "bex_select_device_by_closest_revision: null argument");
+  int n = 0;
+  int rc = bex_device_count(&n);
+  if (rc != BEX_OK) return rc;
+  if (n == 0) return fail(nullptr, BEX_E_HIP, "no HIP device");
+  hipDeviceProp_t want;
+  std::memset(&want, 0, sizeof want);
+  want.major = major;
+  want.minor = minor;
+  const hipError_t e = hipChooseDevice(device, &want);
+  if (e != hipSuccess) return fail(nullptr, BEX_E_HIP, std::string("hipChooseDevice: ") + hipGetErrorString(e));
+  return BEX_OK;
+}
+
 int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   if (!desc || !out) return fail(nullptr, BEX_E_ARGUMENT, "bex_create: null argument");
   *out = nullptr;

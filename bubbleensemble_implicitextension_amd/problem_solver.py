@@ -87,6 +87,32 @@ def compiled_dims():
     return {f: getattr(d, f) for f, _ in d._fields_}
 
 
+# -- device helpers (Interface.cuh:259-345) ----------------------------------------------------
+def list_devices():
+    """Properties of every visible device, as ListCUDADevices prints them (a list of dicts;
+    empty without a device)."""
+    lib = _native.load()
+    n = ctypes.c_int()
+    _native.check(lib.bex_device_count(ctypes.byref(n)))
+    out = []
+    for i in range(n.value):
+        p = _native.DeviceProperties()
+        _native.check(lib.bex_get_device_properties(i, ctypes.byref(p)))
+        d = {f: getattr(p, f) for f, _ in p._fields_}
+        d["name"], d["arch"] = p.name.decode(), p.arch.decode()
+        d["max_threads_dim"], d["max_grid_size"] = list(p.max_threads_dim), list(p.max_grid_size)
+        out.append(d)
+    return out
+
+
+def SelectDeviceByClosestRevision(major, minor):
+    """The device whose architecture revision is closest to major.minor."""
+    dev = ctypes.c_int()
+    _native.check(_native.load().bex_select_device_by_closest_revision(int(major), int(minor),
+                                                                       ctypes.byref(dev)))
+    return dev.value
+
+
 class ProblemSolver:
     """One solver object: pinned host + device buffers for NS systems of UPS units.
 

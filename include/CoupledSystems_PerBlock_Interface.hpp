@@ -71,6 +71,85 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   DenseOutputInterpolation
 };
 
+// --- Device helpers (Interface.cuh:73-75, 259-345) --------------------------------------
+// Same names and output as the reference's free functions, for the HIP devices. The
+// "compute capability" is the architecture revision HIP reports (9.5 for gfx950), followed by
+// the target id.
+
+namespace bex_detail {
+inline void DeviceCheck(int rc, const char* function) {
+  if (rc == BEX_OK) return;
+  std::cerr << "ERROR: In function " << function << "!" << std::endl;
+  std::cerr << "       " << bex_last_error(nullptr) << std::endl;
+  std::exit(EXIT_FAILURE);
+}
+
+inline void PrintDeviceBody(const bex_device_properties& p) {
+  std::cout << "--------------------------------------" << std::endl;
+  std::cout << "Total global memory:        " << p.total_global_mem / 1024 / 1024 << " Mb" << std::endl;
+  std::cout << "Total shared memory:        " << p.shared_mem_per_block / 1024 << " Kb" << std::endl;
+  std::cout << "Number of 32-bit registers: " << p.regs_per_block << std::endl;
+  std::cout << "Total constant memory:      " << p.total_const_mem / 1024 << " Kb" << std::endl;
+  std::cout << std::endl;
+  std::cout << "Number of multiprocessors:  " << p.multiprocessor_count << std::endl;
+  std::cout << "Compute capability:         " << p.major << "." << p.minor << " (" << p.arch << ")"
+            << std::endl;
+  std::cout << "Core clock rate:            " << p.clock_rate_khz / 1000 << " MHz" << std::endl;
+  std::cout << "Memory clock rate:          " << p.memory_clock_rate_khz / 1000 << " MHz" << std::endl;
+  std::cout << "Memory bus width:           " << p.memory_bus_width << " bits" << std::endl;
+  std::cout << "Peak memory bandwidth:      "
+            << 2.0 * p.memory_clock_rate_khz * (p.memory_bus_width / 8) / 1.0e6 << " GB/s"
+            << std::endl;
+  std::cout << std::endl;
+  std::cout << "Warp size:                  " << p.warp_size << std::endl;
+  std::cout << "Max. warps per multiproc:   " << p.max_threads_per_multiprocessor / p.warp_size
+            << std::endl;
+  std::cout << "Max. threads per multiproc: " << p.max_threads_per_multiprocessor << std::endl;
+  std::cout << "Max. threads per block:     " << p.max_threads_per_block << std::endl;
+  std::cout << "Max. block dimensions:      " << p.max_threads_dim[0] << " * "
+            << p.max_threads_dim[1] << " * " << p.max_threads_dim[2] << std::endl;
+  std::cout << "Max. grid dimensions:       " << p.max_grid_size[0] << " * " << p.max_grid_size[1]
+            << " * " << p.max_grid_size[2] << std::endl;
+  std::cout << std::endl;
+  std::cout << "Concurrent memory copy:     " << p.concurrent_copy << std::endl;
+  std::cout << "Execution multiple kernels: " << p.concurrent_kernels << std::endl;
+  std::cout << "ECC support turned on:      " << p.ecc_enabled << std::endl << std::endl;
+}
+}  // namespace bex_detail
+
+inline void ListCUDADevices() {  // Interface.cuh:259-294
+  int n = 0;
+  bex_detail::DeviceCheck(bex_device_count(&n), "ListCUDADevices");
+  for (int i = 0; i < n; i++) {
+    bex_device_properties p;
+    bex_detail::DeviceCheck(bex_get_device_properties(i, &p), "ListCUDADevices");
+    std::cout << std::endl;
+    std::cout << "Device number: " << i << std::endl;
+    std::cout << "Device name:   " << p.name << std::endl;
+    bex_detail::PrintDeviceBody(p);
+  }
+  std::cout << std::endl;
+}
+
+inline int SelectDeviceByClosestRevision(int MajorRevision, int MinorRevision) {  // :296-310
+  int device = 0;
+  bex_detail::DeviceCheck(bex_select_device_by_closest_revision(MajorRevision, MinorRevision, &device),
+                          "SelectDeviceByClosestRevision");
+  std::cout << "HIP Device Number Closest to Revision " << MajorRevision << "." << MinorRevision
+            << ": " << device << std::endl << std::endl << std::endl;
+  return device;
+}
+
+inline void PrintPropertiesOfSpecificDevice(int SelectedDevice) {  // :312-345
+  bex_device_properties p;
+  bex_detail::DeviceCheck(bex_get_device_properties(SelectedDevice, &p),
+                          "PrintPropertiesOfSpecificDevice");
+  std::cout << "Selected device number: " << SelectedDevice << std::endl;
+  std::cout << "Selected device name:   " << p.name << std::endl;
+  bex_detail::PrintDeviceBody(p);
+  std::cout << std::endl;
+}
+
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,
           int NGP, int NiGP, int NUA, int NiUA, int NSA, int NiSA, int NE, int NDO,
           Algorithms Algorithm, class Precision>

@@ -246,6 +246,38 @@ const char* bex_pool_last_error(const bex_pool* p);
  * s is NULL). */
 const char* bex_last_error(const bex_solver* s);
 
+/* Device properties that the reference's device helpers print (Interface.cuh:259-345). */
+typedef struct {
+  char name[256];                   /* device name */
+  char arch[256];                   /* target id, e.g. "gfx950:sramecc+:xnack-" */
+  long long total_global_mem;       /* bytes */
+  long long shared_mem_per_block;   /* LDS bytes available to one workgroup */
+  long long total_const_mem;        /* bytes */
+  int regs_per_block;               /* 32-bit registers per workgroup */
+  int multiprocessor_count;         /* compute units */
+  int major, minor;                 /* architecture revision */
+  int clock_rate_khz;
+  int memory_clock_rate_khz;
+  int memory_bus_width;             /* bits */
+  int warp_size;                    /* wavefront size */
+  int max_threads_per_multiprocessor;
+  int max_threads_per_block;
+  int max_threads_dim[3];
+  int max_grid_size[3];
+  int concurrent_copy;              /* copies overlap kernels */
+  int concurrent_kernels;
+  int ecc_enabled;
+} bex_device_properties;
+
+/* Number of visible devices (0, with BEX_OK, where there is none). Replaces the device loop
+ * of ListCUDADevices (Interface.cuh:259-294). */
+int bex_device_count(int* count);
+/* Properties of one device (cudaGetDeviceProperties in Interface.cuh:264, :314). */
+int bex_get_device_properties(int device, bex_device_properties* out);
+/* The device whose revision is closest to major.minor (SelectDeviceByClosestRevision,
+ * Interface.cuh:296-310). */
+int bex_select_device_by_closest_revision(int major, int minor, int* device);
+
 #ifdef __cplusplus
 }
 #endif

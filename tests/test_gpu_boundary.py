@@ -1,5 +1,6 @@
 """GPU tests of the drop-in boundary beyond the solve itself: the reference's four coupling
-storage schemes and its error conventions.
+storage schemes, its error conventions, its device helpers, and its driver's flow written
+against the C++ header.
 
 Coupling storage (Interface.cuh:1001-1070): the full matrix (CCI = 0, CBW = 0), a band of
 2 CBW + 1 shifted diagonals with circular extension (CCI = 0, CBW > 0), a circulant band
@@ -8,10 +9,14 @@ SetHost accepts exactly the stored elements; the solve must see the matrix those
 define (zeros outside the band), which is checked bit for bit against a full-storage solver
 given that matrix and against the oracle.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 from bubbleensemble_implicitextension_amd import collapse
+from bubbleensemble_implicitextension_amd import problem_solver as ps
 from bubbleensemble_implicitextension_amd import workloads as wl
 from bubbleensemble_implicitextension_amd._native import BexError
 from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
@@ -132,3 +137,78 @@ def test_error_conventions():
     s.SetHost(1, 2, V.ActualState, 1, 0.25)
     assert s.GetHost(1, 2, V.ActualState, 1) == 0.25
     s.close()
+
+
+def test_device_helpers():
+    """ListCUDADevices / SelectDeviceByClosestRevision / PrintPropertiesOfSpecificDevice
+    (Interface.cuh:259-345) report the HIP device."""
+    devs = ps.list_devices()
+    assert len(devs) >= 1
+    d = devs[0]
+    print(f"device 0: {d['name']} {d['arch']} CUs {d['multiprocessor_count']}", flush=True)
+    assert d["arch"].startswith("gfx950")
+    assert d["warp_size"] == 64
+    assert d["multiprocessor_count"] >= 1 and d["total_global_mem"] > 0
+    assert d["max_threads_per_block"] >= 1024 and d["shared_mem_per_block"] >= 64 * 1024
+    assert 0 <= ps.SelectDeviceByClosestRevision(d["major"], d["minor"]) < len(devs)
+    with pytest.raises(BexError):
+        ps._native.check(ps._native.load().bex_get_device_properties(
+            len(devs), ps.ctypes.byref(ps._native.DeviceProperties())))
+
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APP = os.path.join(REPO, "bubbleensemble_implicitextension_amd", "collapse_strength")
+RADII = os.path.join(REPO, "tests", "golden", "BubbleSizes_N32_D1_d150_RE2-8.txt")
+COUPLING = os.path.join(REPO, "tests", "golden", "CouplingMatrix_N32_D1_d150_RE2-8.txt")
+
+
+def app_grid(lo, hi, n):
+    """The driver's linear grid: lo + (i / (n - 1)) (hi - lo), the last point exactly hi."""
+    v = [lo + (i / (n - 1)) * (hi - lo) for i in range(n)]
+    v[-1] = hi
+    return np.array(v)
+
+
+def read_result_file(path):
+    rows = [line.rstrip(",\n").split(",") for line in open(path)]
+    return np.array([[float(x) for x in r] for r in rows])
+
+
+def test_cpp_driver_matches_python_protocol(tmp_path):
+    """The reference driver's flow (CollapseStrength.cu:68-204: device listing, 8 x 8 PA x f grid
+    of 32-bubble systems on the reference's own fixture geometry, 4 transient periods, accessory
+    reset, 32 recording periods, result file) compiled against the drop-in C++ header gives the
+    result file of the Python protocol (collapse.collapse_protocol, write_collapse_file) on the
+    same inputs, to the file's printed precision (the two restate the parameter formulas with
+    different association, a last-bit difference)."""
+    assert os.path.exists(APP), "collapse_strength is not built: run __graft_entry__.build()"
+    pa_lo, pa_hi, f_lo, f_hi = 0.5, 0.8, 100.0, 200.0  # below the bubbles' resonances
+    cpp_out = tmp_path / "cpp.txt"
+    print("C++ driver: 64 systems x 36 periods", flush=True)
+    r = subprocess.run([APP, RADII, COUPLING, str(cpp_out), str(pa_lo), str(pa_hi), str(f_lo),
+                        str(f_hi)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Device number: 0" in r.stdout and "Selected device number: 0" in r.stdout
+
+    re_um = np.loadtxt(RADII)
+    m = np.loadtxt(COUPLING).reshape(32, 32)  # row-major text, row = target bubble
+    f_vals, pa_vals = app_grid(f_lo, f_hi, 8), app_grid(pa_lo, pa_hi, 8)
+    F, PA = np.meshgrid(f_vals, pa_vals, indexing="ij")  # frequency outer, amplitude inner
+    pa, f = PA.reshape(-1), F.reshape(-1)
+    params = wl.unit_parameters(pa[:, None], f[:, None], re_um[None, :])
+    state0 = np.zeros((64, 32, 2))
+    state0[..., 0] = 1.0
+    ens = wl.Ensemble("fixture", 32, pa, f, re_um, params, m, state0, wl.default_options(), 36)
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    print("Python protocol: 64 systems x 36 periods", flush=True)
+    collapse.collapse_protocol(s, 4, 32)
+    py_out = tmp_path / "py.txt"
+    collapse.write_collapse_file(s, str(py_out))
+    s.close()
+
+    a, b = read_result_file(cpp_out), read_result_file(py_out)
+    assert a.shape == b.shape == (64 * 32, 3 + 64)
+    np.testing.assert_array_equal(a[:, :3], b[:, :3])  # system, PA, f
+    np.testing.assert_allclose(a[:, 3:], b[:, 3:], rtol=2e-3, atol=0)
+    assert (a[:, 3:] > 0).mean() > 0.5  # collapses were recorded
