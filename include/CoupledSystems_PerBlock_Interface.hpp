@@ -71,6 +71,36 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   DenseOutputInterpolation
 };
 
+// --- Names of the enumerators (Interface.cuh:70-71, 2035-2106) ----------------------------
+
+inline std::string SolverOptionsToString(ListOfSolverOptions Option) {
+  static const char* const names[] = {
+      "InitialTimeStep",       "ActiveSystems",          "MaximumTimeStep",
+      "MinimumTimeStep",       "TimeStepGrowLimit",      "TimeStepShrinkLimit",
+      "RelativeTolerance",     "AbsoluteTolerance",      "EventTolerance",
+      "EventDirection",        "DenseOutputSaveFrequency", "DenseOutputMinimumTimeStep",
+      "SharedGlobalVariables", "SharedCouplingMatrices", "BiCGRhoTolerance",
+      "BiCGResidualTolerance", "BiCGMaxIterations",      "KernelLayout",
+      "PriorityStepThreshold", "WarmStart",              "DenseOutputInterpolation"};
+  const int i = static_cast<int>(Option) - static_cast<int>(InitialTimeStep);
+  const int n = static_cast<int>(sizeof(names) / sizeof(names[0]));
+  return (i >= 0 && i < n) ? names[i] : "Non-existent solver option!";
+}
+
+inline std::string VariablesToString(ListOfVariables Option) {
+  static const char* const names[] = {
+      "All",            "TimeDomain",        "ActualTime",
+      "ActualState",    "UnitParameters",    "SystemParameters",
+      "GlobalParameters", "IntegerGlobalParameters", "UnitAccessories",
+      "IntegerUnitAccessories", "SystemAccessories", "IntegerSystemAccessories",
+      "CouplingMatrix", "CouplingStrength",  "CouplingIndex",
+      "DenseOutput",    "DenseIndex",        "DenseTime",
+      "DenseState"};
+  const int i = static_cast<int>(Option) - static_cast<int>(All);
+  const int n = static_cast<int>(sizeof(names) / sizeof(names[0]));
+  return (i >= 0 && i < n) ? names[i] : "Non-existent variable!";
+}
+
 // --- Device helpers (Interface.cuh:73-75, 259-345) --------------------------------------
 // Same names and output as the reference's free functions, for the HIP devices. The
 // "compute capability" is the architecture revision HIP reports (9.5 for gfx950), followed by
