@@ -297,8 +297,8 @@ __device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, doubl
       }
     } else {
       const double beta = rho1 / rho2;
-      p = r + beta * p;
-      pt = rt + beta * pt;
+      p = fma(beta, p, r);
+      pt = fma(beta, pt, rt);
     }
     lay.sync();
     vec[u] = make_double2(x1sq * p, rD * pt);
@@ -309,9 +309,9 @@ __device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, doubl
     const double qt = fma(x1sq, a2, pt);
     out.iterations++;
     const double alpha = rho1 / lay.sum(pt * q);
-    x = x + alpha * p;
-    r = r - alpha * q;
-    rt = rt - alpha * qt;
+    x = fma(alpha, p, x);
+    r = fma(-alpha, q, r);
+    rt = fma(-alpha, qt, rt);
     rho2 = rho1;
     const double rmax = lay.fmax_all(bex::fabs_d(r));
     rho1 = lay.sum(r * rt);
@@ -788,7 +788,9 @@ __device__ __forceinline__ double rp_residual(const L& lay, double b, double x0,
 // BiCG for A x = b from x0 (Steppers.cuh:494-561 with x0 in place of xu): x on return is
 // meaningful on the primal half. The loop is rotated: the reference's iteration-1 branch (the
 // rho exit, p = r) runs before it and the direction update p = r + beta p of iteration k + 1
-// closes iteration k, which leaves the arithmetic and its order unchanged. Scalars (rho,
+// closes iteration k, which leaves the arithmetic and its order unchanged. The vector updates
+// x + alpha p, r - alpha q and r + beta p are single fmas (one rounding, as nvcc contracts
+// them in the reference's kernel), in the oracle's canonical mode too. Scalars (rho,
 // alpha, beta, max |r|) are identical in every lane, so the branches on them are uniform.
 template <class L>
 __device__ __forceinline__ BicgOut rp_bicg(const L& lay, double res0, double x0, double x1sq,
@@ -811,8 +813,8 @@ __device__ __forceinline__ BicgOut rp_bicg(const L& lay, double res0, double x0,
         const double qv = fma(s2, lay.matvec(s1 * dir), dir);  // q (primal), q~ (shadow)
         out.iterations++;
         const double alpha = rho1 / lay.dot_halves(primal ? qv : dir);  // p~ . q
-        x = x + alpha * dir;
-        res = res - alpha * qv;
+        x = fma(alpha, dir, x);
+        res = fma(-alpha, qv, res);
         const double rho2 = rho1;
         // The residual test (max |r| < tolerance) and the next r . r~.
         if (lay.res_test(res, a.BiCGResidualTolerance, &rho1)) {
@@ -824,7 +826,7 @@ __device__ __forceinline__ BicgOut rp_bicg(const L& lay, double res0, double x0,
           break;
         }
         const double beta = rho1 / rho2;
-        dir = res + beta * dir;
+        dir = fma(beta, dir, res);
       }
     }
   }

@@ -230,8 +230,13 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
     } else {                                                           // :526-531
       double beta = rho1 / rho2;
       for (int i = 0; i < N; i++) {
-        w.p[i] = w.r[i] + beta * w.p[i];
-        w.pt[i] = w.rt[i] + beta * w.pt[i];
+        if (MODE == BEX_ORACLE_LITERAL) {
+          w.p[i] = w.r[i] + beta * w.p[i];
+          w.pt[i] = w.rt[i] + beta * w.pt[i];
+        } else {  // one rounding, as nvcc contracts r + beta * p
+          w.p[i] = std::fma(beta, w.p[i], w.r[i]);
+          w.pt[i] = std::fma(beta, w.pt[i], w.rt[i]);
+        }
       }
     }
     // q = A p, qt = A^T pt                                            :536, :669-706
@@ -261,9 +266,15 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
     for (int i = 0; i < N; i++) w.tmp[i] = w.pt[i] * w.q[i];
     double alpha = rho1 / tree_reduce(w.tmp, N, AddOp());              // :539-541
     for (int i = 0; i < N; i++) {                                      // :545-548
-      x[i] = x[i] + alpha * w.p[i];
-      w.r[i] = w.r[i] - alpha * w.q[i];
-      w.rt[i] = w.rt[i] - alpha * w.qt[i];
+      if (MODE == BEX_ORACLE_LITERAL) {
+        x[i] = x[i] + alpha * w.p[i];
+        w.r[i] = w.r[i] - alpha * w.q[i];
+        w.rt[i] = w.rt[i] - alpha * w.qt[i];
+      } else {  // one rounding each, as nvcc contracts them
+        x[i] = std::fma(alpha, w.p[i], x[i]);
+        w.r[i] = std::fma(-alpha, w.q[i], w.r[i]);
+        w.rt[i] = std::fma(-alpha, w.qt[i], w.rt[i]);
+      }
       w.tmp[i] = std::fabs(w.r[i]);
     }
     rho2 = rho1;                                                       // :551-552
