@@ -125,11 +125,36 @@ def run_leg(ens, steps, warmup, end_time, network=None):
     wall = time.perf_counter() - t0
 
     solver.SynchroniseFromDeviceToHost(V.All)
+    solver.InsertSynchronisationPoint()  # the copies are asynchronous on the solver's stream
     solver.SynchroniseSolver()
     out = {"wall": wall, "kernel_ms": kernel_ms, "stats": solver.stats(),
            "state": solver.state().copy(), "acc": solver.accessories().copy(),
            "events": solver.event_counts().copy()}
     solver.close()
+    return out
+
+
+def run_learned_batched_leg(ens, steps, warmup, end_time, network, heavy):
+    """Times `steps` periods of the ensemble split by cost: the `heavy` costliest systems
+    (forcing amplitude over frequency) with the learned start, the rest with the plain start,
+    both solver objects launched together (inputs resident on the device)."""
+    from bubbleensemble_implicitextension_amd.warm_start import LearnedBatchedSolver
+
+    lb = LearnedBatchedSolver(ens, network, heavy=heavy, end_time=end_time)
+    for _ in range(warmup):
+        lb.Solve()
+    lb.synchronise()
+    lb.reset_stats()
+    lb.synchronise()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lb.Solve()
+    lb.synchronise()
+    wall = time.perf_counter() - t0
+    out = lb.results()
+    out["wall"] = wall
+    out["heavy"] = lb.heavy
+    lb.close()
     return out
 
 
@@ -218,6 +243,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm-start leg")
+    ap.add_argument("--heavy", type=int, default=64,
+                    help="systems on the learned start in the learned+batched leg")
     ap.add_argument("--pool-end-time", type=float, default=0.1,
                     help="period fraction of the config-2 pool leg (0 skips it)")
     ap.add_argument("--end-time", type=float, default=1.0,
@@ -252,7 +279,8 @@ def main():
 
     line = {
         "metric": "systems x acoustic periods/s at rtol=atol=1e-10, plain FP64 RKCK45 + BiCG "
-                  "(learned warm start vs plain in 'learned'); BiCG iters/RHS",
+                  "(learned warm start vs plain in 'learned', learned+batched in "
+                  "'learned_batched'); BiCG iters/RHS",
         "value": value,
         "unit": "system-periods/s",
         "n_gpus": 1,
@@ -292,6 +320,25 @@ def main():
             "network": "5-64-64-1 BF16 MFMA, trained on a held-out-seed ensemble",
             "error_vs_plain": error_vs(plain, learned),
         }
+    if not args.no_learned and os.path.exists(warm_start.DEFAULT_NETWORK):
+        net = np.load(warm_start.DEFAULT_NETWORK)
+        lbr = run_learned_batched_leg(ens, args.steps, args.warmup, args.end_time, net,
+                                      args.heavy)
+        lbs = lbr["stats"]
+        lbv = ens.NS * args.steps / lbr["wall"]
+        line["learned_batched"] = {
+            "value": lbv, "unit": "system-periods/s",
+            "ms_per_step": lbr["wall"] * 1e3 / args.steps,
+            "speedup_vs_plain": lbv / value,
+            "split": f"{len(lbr['heavy'])} systems with the largest forcing amplitude / "
+                     "frequency on the learned start, the others on the plain start, two "
+                     "solver objects launched together",
+            "learned_systems": [int(k) for k in lbr["heavy"][:8]] + (["..."] if len(lbr["heavy"]) > 8 else []),
+            "bicg_iterations_per_rhs": float(lbs[:, 3].sum()) / max(1.0, float(lbs[:, 2].sum())),
+            "error_vs_plain": error_vs(plain, lbr),
+        }
+        print(f"# learned+batched leg done ({lbr['wall'] * 1e3:.1f} ms)", file=sys.stderr,
+              flush=True)
     if args.pool_end_time > 0:
         line["pool"] = run_pool_leg(args.pool_end_time)
         print(f"# pool leg done ({line['pool']['ms']:.0f} ms)", file=sys.stderr, flush=True)

@@ -109,6 +109,7 @@ def _run(ens, network=None, res_tol=1e-13, maxit=60):
     s.SynchroniseFromHostToDevice(V.All)
     s.Solve()
     s.SynchroniseFromDeviceToHost(V.All)
+    s.InsertSynchronisationPoint()  # the copies are asynchronous on the solver's stream
     s.SynchroniseSolver()
     out = (s.state().copy(), s.accessories().copy(), s.event_counts().copy(), s.stats().copy())
     s.close()
@@ -167,3 +168,44 @@ def test_learned_warm_start_ablation():
     rel = np.abs(xl - xp) / np.maximum(np.abs(xp), 1e-3)
     assert np.max(rel) <= 1e-6, np.max(rel)
     assert it_learned <= 1.25 * it_plain
+
+
+def test_learned_batched_split():
+    """Learned + batched: the costliest systems by the amplitude/frequency proxy run with the
+    learned start, the others with the plain start, both launched together; the plain-start
+    systems are bit-identical to a plain solve of the whole ensemble and the learned-start
+    systems stay within the learned start's stated error."""
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd import warm_start as wsm
+    from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    ens = wl.config1().subset(np.arange(0, 16384, 64))  # 256 systems over the whole grid
+    net = np.load(wsm.DEFAULT_NETWORK)
+    print(f"case: split of {ens.NS} systems, 16 on the learned start, tau in [0, 0.3]", flush=True)
+    lb = wsm.LearnedBatchedSolver(ens, net, heavy=16, end_time=0.3)
+    lb.Solve()
+    lb.synchronise()
+    got = lb.results()
+    heavy, light = lb.heavy, lb.light
+    lb.close()
+    want_heavy = np.sort(np.argsort(-wsm.cost_proxy(ens), kind="stable")[:16])
+    np.testing.assert_array_equal(heavy, want_heavy)
+
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.host_view(V.TimeDomain)[ens.NS:] = 0.3
+    s.SynchroniseFromHostToDevice(V.All)
+    s.Solve()
+    s.SynchroniseFromDeviceToHost(V.All)
+    s.InsertSynchronisationPoint()
+    s.SynchroniseSolver()
+    state, acc, ev, st = s.state(), s.accessories(), s.event_counts(), s.stats()
+    for name, a, b in (("state", got["state"], state), ("acc", got["acc"], acc)):
+        np.testing.assert_array_equal(a[light].view(np.int64), b[light].view(np.int64), name)
+    np.testing.assert_array_equal(got["events"][light], ev[light])
+    np.testing.assert_array_equal(got["stats"][light], st[light])
+    assert np.all(np.isfinite(got["state"][heavy]))
+    assert np.all(got["stats"][heavy, 0] > 0)  # the learned-start systems stepped
+    assert np.mean(got["events"][heavy] == ev[heavy]) >= 0.99
+    s.close()
