@@ -67,7 +67,7 @@ def cpu_baseline(sample=2048, seed=7):
                       f"workload, first period from rest, literal host restatement, {threads} "
                       f"OpenMP threads, {dt:.1f} s",
             "single_thread_value": len(one_sids) / dt1,
-            "single_thread_sample": f"every {max(1, sample // 64)}th system of that sample "
+            "single_thread_sample": f"one in {max(1, sample // 64)} systems of that sample "
                                     f"({len(one_sids)} systems), 1 thread, {dt1:.1f} s"}
 
 
