@@ -362,12 +362,18 @@ BEX_HD double exp_dd(double th, double tl) {
   double res = 1.0 + em1;
   const int ni = (int)n;
   // Two-step scaling keeps 2^ni representable for results near the overflow and underflow
-  // thresholds; for |ni| <= 1000 the second step is a scaling by 2^0.
+  // thresholds; for |ni| <= 1000 the second step is a scaling by 2^0 and is skipped, as are the
+  // special values, behind one branch that ordinary arguments never take (same values).
   const int n1 = ni > 1000 ? 1000 : (ni < -1000 ? -1000 : ni);
-  double out = ldexp(ldexp(res, n1), ni - n1);
-  out = over ? infinity() : out;
-  out = under ? 0.0 : out;
-  return nan ? quiet_nan() : out;
+  double out = ldexp(res, n1);
+  if (!regular || n1 != ni) {
+    asm volatile("");  // a branch, not selects evaluated for every argument
+    out = ldexp(out, ni - n1);
+    out = over ? infinity() : out;
+    out = under ? 0.0 : out;
+    out = nan ? quiet_nan() : out;
+  }
+  return out;
 }
 
 BEX_HD bool is_integer(double y) { return rint(y) == y; }
@@ -394,15 +400,20 @@ BEX_HD double pow(double x, double y) {
   const double th = ym * lh;
   const double tl = fma(ym, lh, -th) + ym * ll;
   double r = sign * exp_dd(th, tl);
-  // Special cases, lowest precedence first.
-  r = !finite_x ? ((y > 0.0) ? sign * infinity() : sign * 0.0) : r;
-  r = (ax == 0.0) ? ((y > 0.0) ? sign * 0.0 : sign * infinity()) : r;
-  r = (negative && (x != 0.0) && finite_x && !is_integer(y)) ? quiet_nan() : r;
-  const double at_inf = (ax == 1.0) ? 1.0 : (((ax > 1.0) == (y > 0.0)) ? infinity() : 0.0);
-  r = !finite_y ? at_inf : r;
-  r = (is_nan(x) || is_nan(y)) ? quiet_nan() : r;
-  r = (x == 1.0) ? 1.0 : r;
-  r = (y == 0.0) ? 1.0 : r;
+  // Special cases, lowest precedence first. On a finite nonzero x and a finite y, other than a
+  // negative x with a non-integer y, none of them changes r (x = 1 and y = 0 give exactly 1 on
+  // the main path), so they sit behind one branch that ordinary arguments never take.
+  if (!regular || (negative && !is_integer(y))) {
+    asm volatile("");  // a branch, not selects evaluated for every argument
+    r = !finite_x ? ((y > 0.0) ? sign * infinity() : sign * 0.0) : r;
+    r = (ax == 0.0) ? ((y > 0.0) ? sign * 0.0 : sign * infinity()) : r;
+    r = (negative && (x != 0.0) && finite_x && !is_integer(y)) ? quiet_nan() : r;
+    const double at_inf = (ax == 1.0) ? 1.0 : (((ax > 1.0) == (y > 0.0)) ? infinity() : 0.0);
+    r = !finite_y ? at_inf : r;
+    r = (is_nan(x) || is_nan(y)) ? quiet_nan() : r;
+    r = (x == 1.0) ? 1.0 : r;
+    r = (y == 0.0) ? 1.0 : r;
+  }
   return r;
 }
 
