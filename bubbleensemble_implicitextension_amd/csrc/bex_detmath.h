@@ -196,11 +196,13 @@ BEX_HD double cos_tail(double r2, const Coef& C) {
 }
 
 // ---------------------------------------------------------------------------------------
-// The functions below evaluate their main path on sanitised arguments and select the special
+// The functions below evaluate their main path on sanitised arguments and apply the special
 // cases (non-finite input, integer-valued sincospi argument, pow's C99 cases, exp's overflow
 // and underflow) at the end, instead of returning early: on the device an early return is a
-// divergent branch (exec-mask bookkeeping on every call), a select is one instruction per half.
-// The values are those of the same formulas with early returns, for every input.
+// divergent branch (exec-mask bookkeeping on every call). sincospi and sincos select them (a
+// select is one instruction per half); pow and exp_dd, whose chains are longer, put them
+// behind one branch that ordinary arguments never take. The values are those of the same
+// formulas with early returns, for every input.
 
 // Quadrant selection: (s, c) = (sin, cos) of r + q pi/2 from (sin r, cos r) = (sr, cr).
 BEX_HD void quadrant(int q, double sr, double cr, double* s, double* c) {
