@@ -199,6 +199,8 @@ class LearnedBatchedSolver:
     """
 
     def __init__(self, ens, network, heavy=64, end_time=1.0):
+        if ens.NS < 2:
+            raise ValueError("a learned + batched split needs at least two systems")
         order = np.argsort(-cost_proxy(ens), kind="stable")
         heavy = max(1, min(int(heavy), ens.NS - 1))
         self.heavy = np.sort(order[:heavy])
