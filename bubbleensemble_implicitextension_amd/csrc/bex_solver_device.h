@@ -1704,8 +1704,9 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     constexpr int NSTAGE = (ALG == kRKCK45) ? 6 : 4;
     // The final pass of interpolated dense output evaluates the first stage only.
     const int nstage = (interp && S.flushOnly != 0) ? 1 : NSTAGE;
-    // One copy of the right-hand side + BiCG body: the stage loop is not unrolled and each
-    // stage's input state is formed in the reference's association (Steppers.cuh:153-400).
+    // One copy of the right-hand side + BiCG body: the stage loop is not unrolled. Each stage's
+    // input is the reference's sum (Steppers.cuh:153-400) contracted as nvcc contracts it: a
+    // multiply, one fma per further term, and x + sum * dt as one fma; likewise the stage time.
 #pragma nounroll
     for (int s = 0; s < nstage; s++) {
       double xs[UD], F[UD];
@@ -1722,33 +1723,34 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
             break;
           case 1:
 #pragma unroll
-            for (int i = 0; i < UD; i++) xs[i] = x[i] + (k1[i] * A[0]) * dt;
-            Ts = t + A[5] * dt;
+            for (int i = 0; i < UD; i++) xs[i] = fma(k1[i] * A[0], dt, x[i]);
+            Ts = fma(A[5], dt, t);
             break;
           case 2:
 #pragma unroll
-            for (int i = 0; i < UD; i++) xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1]) * dt;
-            Ts = t + A[5] * dt;
+            for (int i = 0; i < UD; i++) xs[i] = fma(fma(k2[i], A[1], k1[i] * A[0]), dt, x[i]);
+            Ts = fma(A[5], dt, t);
             break;
           case 3:
 #pragma unroll
             for (int i = 0; i < UD; i++)
-              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2]) * dt;
-            Ts = t + A[5] * dt;
+              xs[i] = fma(fma(k3[i], A[2], fma(k2[i], A[1], k1[i] * A[0])), dt, x[i]);
+            Ts = fma(A[5], dt, t);
             break;
           case 4:
 #pragma unroll
             for (int i = 0; i < UD; i++)
-              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2] + k4[i] * A[3]) * dt;
+              xs[i] = fma(fma(k4[i], A[3], fma(k3[i], A[2], fma(k2[i], A[1], k1[i] * A[0]))),
+                          dt, x[i]);
             Ts = t + dt;
             break;
           default:
 #pragma unroll
             for (int i = 0; i < UD; i++)
-              xs[i] = x[i] + (k1[i] * A[0] + k2[i] * A[1] + k3[i] * A[2] + k4[i] * A[3] +
-                              k5[i] * A[4]) *
-                                 dt;
-            Ts = t + A[5] * dt;
+              xs[i] = fma(fma(k5[i], A[4],
+                              fma(k4[i], A[3], fma(k3[i], A[2], fma(k2[i], A[1], k1[i] * A[0])))),
+                          dt, x[i]);
+            Ts = fma(A[5], dt, t);
             break;
         }
       } else {
@@ -1756,9 +1758,9 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         for (int i = 0; i < UD; i++) {
           switch (s) {
             case 0: xs[i] = x[i]; break;
-            case 1: xs[i] = x[i] + k1[i] * (0.5 * dt); break;
-            case 2: xs[i] = x[i] + k2[i] * (0.5 * dt); break;
-            default: xs[i] = x[i] + k3[i] * dt; break;
+            case 1: xs[i] = fma(k1[i], 0.5 * dt, x[i]); break;
+            case 2: xs[i] = fma(k2[i], 0.5 * dt, x[i]); break;
+            default: xs[i] = fma(k3[i], dt, x[i]); break;
           }
         }
         Ts = (s == 0) ? t : ((s == 3) ? t + dt : t + 0.5 * dt);
@@ -1811,20 +1813,24 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     if (ALG == kRKCK45) {
 #pragma unroll
       for (int i = 0; i < UD; i++) {
-        xn[i] = x[i] + (k1[i] * (37.0 / 378.0) + k3[i] * (250.0 / 621.0) +
-                        k4[i] * (125.0 / 594.0) + k6[i] * (512.0 / 1771.0)) *
-                           dt;
-        double e = k1[i] * (37.0 / 378.0 - 2825.0 / 27648.0) +
-                   k3[i] * (250.0 / 621.0 - 18575.0 / 48384.0) +
-                   k4[i] * (125.0 / 594.0 - 13525.0 / 55296.0) +
-                   k5[i] * (0.0 - 277.0 / 14336.0) + k6[i] * (512.0 / 1771.0 - 1.0 / 4.0);
-        err[i] = dt * bex::fabs_d(e) + 1e-18;
+        // Steppers.cuh:408-418, contracted like the stage inputs.
+        double sum = k1[i] * (37.0 / 378.0);
+        sum = fma(k3[i], 250.0 / 621.0, sum);
+        sum = fma(k4[i], 125.0 / 594.0, sum);
+        sum = fma(k6[i], 512.0 / 1771.0, sum);
+        xn[i] = fma(sum, dt, x[i]);
+        double e = k1[i] * (37.0 / 378.0 - 2825.0 / 27648.0);
+        e = fma(k3[i], 250.0 / 621.0 - 18575.0 / 48384.0, e);
+        e = fma(k4[i], 125.0 / 594.0 - 13525.0 / 55296.0, e);
+        e = fma(k5[i], 0.0 - 277.0 / 14336.0, e);
+        e = fma(k6[i], 512.0 / 1771.0 - 1.0 / 4.0, e);
+        err[i] = fma(dt, bex::fabs_d(e), 1e-18);
         if (!bex::is_finite(xn[i]) || !bex::is_finite(err[i])) finiteLane = false;
       }
     } else {
 #pragma unroll
       for (int i = 0; i < UD; i++) {
-        xn[i] = x[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) * (dt * (1.0 / 6.0));
+        xn[i] = fma(k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i], dt * (1.0 / 6.0), x[i]);
         err[i] = 0.0;
         if (!bex::is_finite(xn[i])) finiteLane = false;
       }

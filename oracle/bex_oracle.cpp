@@ -19,6 +19,8 @@
 #include "bex_oracle.h"
 
 #include <cmath>
+#include <initializer_list>
+#include <utility>
 #include <cstring>
 #include <vector>
 
@@ -287,7 +289,32 @@ BicgResult bicg(int N, const double* M, const double* x1, const double* rD, cons
 // ---------------------------------------------------------------------------------------
 // Cash-Karp RKCK45 step (Steppers.cuh:153-424) for n values with stage(s, Ts) filling K[s].
 // Returns 0 if the new state or the error estimate is not finite (Steppers.cuh:420-421).
-template <class Stage>
+// Canonical mode: every sum of products is one multiply followed by fmas and every x + c dt
+// one fma, the contraction nvcc applies to these lines of the reference (Steppers.cuh:197-418);
+// literal mode keeps separate multiplies and adds.
+template <int MODE>
+double rk_sum(std::initializer_list<std::pair<double, double>> terms) {
+  double s = 0.0;
+  bool first = true;
+  for (const auto& kc : terms) {
+    if (MODE == BEX_ORACLE_LITERAL)
+      s = first ? kc.first * kc.second : s + kc.first * kc.second;
+    else
+      s = first ? kc.first * kc.second : std::fma(kc.first, kc.second, s);
+    first = false;
+  }
+  return s;
+}
+template <int MODE>
+double rk_axpy(double x, double sum, double h) {  // x + sum * h
+  return MODE == BEX_ORACLE_LITERAL ? x + sum * h : std::fma(sum, h, x);
+}
+template <int MODE>
+double rk_time(double t, double c, double dt) {  // t + c * dt
+  return MODE == BEX_ORACLE_LITERAL ? t + c * dt : std::fma(c, dt, t);
+}
+
+template <int MODE = BEX_ORACLE_LITERAL, class Stage>
 int rkck45_step(int n, const double* X, double t, double dt, double* K, double* Xs, double* Xn,
                 double* Err, Stage stage) {
   double* k1 = &K[0];
@@ -299,33 +326,40 @@ int rkck45_step(int n, const double* X, double t, double dt, double* K, double* 
   int isFinite = 1;
   for (int i = 0; i < n; i++) Xs[i] = X[i];
   stage(0, t);
-  for (int i = 0; i < n; i++) Xs[i] = X[i] + (k1[i] * (1.0 / 5.0)) * dt;
-  stage(1, t + (1.0 / 5.0) * dt);
-  for (int i = 0; i < n; i++) Xs[i] = X[i] + (k1[i] * (3.0 / 40.0) + k2[i] * (9.0 / 40.0)) * dt;
-  stage(2, t + (3.0 / 10.0) * dt);
+  for (int i = 0; i < n; i++) Xs[i] = rk_axpy<MODE>(X[i], rk_sum<MODE>({{k1[i], 1.0 / 5.0}}), dt);
+  stage(1, rk_time<MODE>(t, 1.0 / 5.0, dt));
   for (int i = 0; i < n; i++)
-    Xs[i] = X[i] + (k1[i] * (3.0 / 10.0) + k2[i] * (-9.0 / 10.0) + k3[i] * (6.0 / 5.0)) * dt;
-  stage(3, t + (3.0 / 5.0) * dt);
+    Xs[i] = rk_axpy<MODE>(X[i], rk_sum<MODE>({{k1[i], 3.0 / 40.0}, {k2[i], 9.0 / 40.0}}), dt);
+  stage(2, rk_time<MODE>(t, 3.0 / 10.0, dt));
   for (int i = 0; i < n; i++)
-    Xs[i] = X[i] + (k1[i] * (-11.0 / 54.0) + k2[i] * (5.0 / 2.0) + k3[i] * (-70.0 / 27.0) +
-                    k4[i] * (35.0 / 27.0)) *
-                       dt;
+    Xs[i] = rk_axpy<MODE>(
+        X[i], rk_sum<MODE>({{k1[i], 3.0 / 10.0}, {k2[i], -9.0 / 10.0}, {k3[i], 6.0 / 5.0}}), dt);
+  stage(3, rk_time<MODE>(t, 3.0 / 5.0, dt));
+  for (int i = 0; i < n; i++)
+    Xs[i] = rk_axpy<MODE>(X[i],
+                          rk_sum<MODE>({{k1[i], -11.0 / 54.0}, {k2[i], 5.0 / 2.0},
+                                        {k3[i], -70.0 / 27.0}, {k4[i], 35.0 / 27.0}}),
+                          dt);
   stage(4, t + dt);
   for (int i = 0; i < n; i++)
-    Xs[i] = X[i] + (k1[i] * (1631.0 / 55296.0) + k2[i] * (175.0 / 512.0) +
-                    k3[i] * (575.0 / 13824.0) + k4[i] * (44275.0 / 110592.0) +
-                    k5[i] * (253.0 / 4096.0)) *
-                       dt;
-  stage(5, t + (7.0 / 8.0) * dt);
+    Xs[i] = rk_axpy<MODE>(X[i],
+                          rk_sum<MODE>({{k1[i], 1631.0 / 55296.0}, {k2[i], 175.0 / 512.0},
+                                        {k3[i], 575.0 / 13824.0}, {k4[i], 44275.0 / 110592.0},
+                                        {k5[i], 253.0 / 4096.0}}),
+                          dt);
+  stage(5, rk_time<MODE>(t, 7.0 / 8.0, dt));
   for (int i = 0; i < n; i++) {
-    Xn[i] = X[i] + (k1[i] * (37.0 / 378.0) + k3[i] * (250.0 / 621.0) + k4[i] * (125.0 / 594.0) +
-                    k6[i] * (512.0 / 1771.0)) *
-                       dt;
-    double e = k1[i] * (37.0 / 378.0 - 2825.0 / 27648.0) +
-               k3[i] * (250.0 / 621.0 - 18575.0 / 48384.0) +
-               k4[i] * (125.0 / 594.0 - 13525.0 / 55296.0) + k5[i] * (0.0 - 277.0 / 14336.0) +
-               k6[i] * (512.0 / 1771.0 - 1.0 / 4.0);
-    Err[i] = dt * std::fabs(e) + 1e-18;
+    Xn[i] = rk_axpy<MODE>(X[i],
+                          rk_sum<MODE>({{k1[i], 37.0 / 378.0}, {k3[i], 250.0 / 621.0},
+                                        {k4[i], 125.0 / 594.0}, {k6[i], 512.0 / 1771.0}}),
+                          dt);
+    double e = rk_sum<MODE>({{k1[i], 37.0 / 378.0 - 2825.0 / 27648.0},
+                             {k3[i], 250.0 / 621.0 - 18575.0 / 48384.0},
+                             {k4[i], 125.0 / 594.0 - 13525.0 / 55296.0},
+                             {k5[i], 0.0 - 277.0 / 14336.0},
+                             {k6[i], 512.0 / 1771.0 - 1.0 / 4.0}});
+    Err[i] = MODE == BEX_ORACLE_LITERAL ? dt * std::fabs(e) + 1e-18
+                                        : std::fma(dt, std::fabs(e), 1e-18);
     if (!std::isfinite(Xn[i]) || !std::isfinite(Err[i])) isFinite = 0;
   }
   return isFinite;
@@ -333,7 +367,7 @@ int rkck45_step(int n, const double* X, double t, double dt, double* K, double* 
 
 // Classical RK4 step. The reference declares RK4 (Interface.cuh:33) and its controller
 // (ErrorControllers.cuh:424-450) but has no per-block RK4 stepper.
-template <class Stage>
+template <int MODE = BEX_ORACLE_LITERAL, class Stage>
 int rk4_step(int n, const double* X, double t, double dt, double* K, double* Xs, double* Xn,
              Stage stage) {
   double* k1 = &K[0];
@@ -343,14 +377,14 @@ int rk4_step(int n, const double* X, double t, double dt, double* K, double* Xs,
   int isFinite = 1;
   for (int i = 0; i < n; i++) Xs[i] = X[i];
   stage(0, t);
-  for (int i = 0; i < n; i++) Xs[i] = X[i] + k1[i] * (0.5 * dt);
+  for (int i = 0; i < n; i++) Xs[i] = rk_axpy<MODE>(X[i], k1[i], 0.5 * dt);
   stage(1, t + 0.5 * dt);
-  for (int i = 0; i < n; i++) Xs[i] = X[i] + k2[i] * (0.5 * dt);
+  for (int i = 0; i < n; i++) Xs[i] = rk_axpy<MODE>(X[i], k2[i], 0.5 * dt);
   stage(2, t + 0.5 * dt);
-  for (int i = 0; i < n; i++) Xs[i] = X[i] + k3[i] * dt;
+  for (int i = 0; i < n; i++) Xs[i] = rk_axpy<MODE>(X[i], k3[i], dt);
   stage(3, t + dt);
   for (int i = 0; i < n; i++) {
-    Xn[i] = X[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) * (dt * (1.0 / 6.0));
+    Xn[i] = rk_axpy<MODE>(X[i], k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i], dt * (1.0 / 6.0));
     if (!std::isfinite(Xn[i])) isFinite = 0;
   }
   return isFinite;
@@ -540,7 +574,8 @@ void solve_system(const Ctx& c, int sid) {
 
     const int n2 = N * UD;
     if (o.Algorithm == 1) {
-      isFinite = rkck45_step(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), Err.data(), stage);
+      isFinite = rkck45_step<MODE>(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), Err.data(),
+                                   stage);
       // Error controller reductions (ErrorControllers.cuh:477-497)
       double relErr = 1e30;
       for (int u = 0; u < N; u++) {
@@ -566,7 +601,7 @@ void solve_system(const Ctx& c, int sid) {
       }
       if (cc.forced) st[BEX_STAT_FORCED_ACCEPTS]++;
     } else {
-      isFinite = rk4_step(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), stage);
+      isFinite = rk4_step<MODE>(n2, X.data(), t, dt, K.data(), Xs.data(), Xn.data(), stage);
       if (isFinite == 0) {  // ErrorControllers.cuh:434-446
         terminate = 1;
         updateStep = 0;
