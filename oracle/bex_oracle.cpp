@@ -781,21 +781,25 @@ extern "C" double bex_oracle_cos(double x, int mode) {
   return mode == BEX_ORACLE_LITERAL ? MathLiteral::cos(x) : MathCanonical::cos(x);
 }
 
-extern "C" int bex_oracle_rkck45_linear(double lambda, double y0, double h, double* y5,
+extern "C" int bex_oracle_rkck45_linear(double lambda, double y0, double h, int mode, double* y5,
                                         double* err) {
-  // y' = lambda*y through the oracle's RKCK45 tableau (unit dimension 1).
+  // y' = lambda*y through the oracle's RKCK45 tableau (unit dimension 1), in either arithmetic.
   double X[1] = {y0}, K[6], Xs[1], Xn[1], Err[1];
   auto stage = [&](int s, double) { K[s] = lambda * Xs[0]; };
-  int ok = rkck45_step(1, X, 0.0, h, K, Xs, Xn, Err, stage);
+  int ok = mode == BEX_ORACLE_LITERAL
+               ? rkck45_step<BEX_ORACLE_LITERAL>(1, X, 0.0, h, K, Xs, Xn, Err, stage)
+               : rkck45_step<BEX_ORACLE_CANONICAL>(1, X, 0.0, h, K, Xs, Xn, Err, stage);
   *y5 = Xn[0];
   *err = Err[0];
   return ok;
 }
 
-extern "C" int bex_oracle_rk4_linear(double lambda, double y0, double h, double* y) {
+extern "C" int bex_oracle_rk4_linear(double lambda, double y0, double h, int mode, double* y) {
   double X[1] = {y0}, K[4], Xs[1], Xn[1];
   auto stage = [&](int s, double) { K[s] = lambda * Xs[0]; };
-  int ok = rk4_step(1, X, 0.0, h, K, Xs, Xn, stage);
+  int ok = mode == BEX_ORACLE_LITERAL
+               ? rk4_step<BEX_ORACLE_LITERAL>(1, X, 0.0, h, K, Xs, Xn, stage)
+               : rk4_step<BEX_ORACLE_CANONICAL>(1, X, 0.0, h, K, Xs, Xn, stage);
   *y = Xn[0];
   return ok;
 }

@@ -105,8 +105,9 @@ void bex_oracle_km_rhs(const double* X, double T, const double* uPAR, int mode, 
                        double* CPT, double* CPF);
 
 /* The oracle's RKCK45 / RK4 tableaux applied to y' = lambda*y for one step (tests). */
-int bex_oracle_rkck45_linear(double lambda, double y0, double h, double* y5, double* err);
-int bex_oracle_rk4_linear(double lambda, double y0, double h, double* y);
+int bex_oracle_rkck45_linear(double lambda, double y0, double h, int mode, double* y5,
+                             double* err);
+int bex_oracle_rk4_linear(double lambda, double y0, double h, int mode, double* y);
 
 /* The RKCK45 step-size controller after its reductions (tests). */
 void bex_oracle_control(int updateStep, int isFinite, double relErr, double dt,

@@ -101,9 +101,9 @@ def lib():
         L.bex_oracle_sin.restype = ctypes.c_double
         L.bex_oracle_cos.argtypes = [ctypes.c_double, ctypes.c_int]
         L.bex_oracle_cos.restype = ctypes.c_double
-        L.bex_oracle_rkck45_linear.argtypes = [ctypes.c_double] * 3 + [_DP, _DP]
+        L.bex_oracle_rkck45_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP, _DP]
         L.bex_oracle_rkck45_linear.restype = ctypes.c_int
-        L.bex_oracle_rk4_linear.argtypes = [ctypes.c_double] * 3 + [_DP]
+        L.bex_oracle_rk4_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP]
         L.bex_oracle_rk4_linear.restype = ctypes.c_int
         L.bex_oracle_control.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.POINTER(Options), ctypes.c_int,
@@ -237,16 +237,16 @@ def km_rhs(X, T, upar, mode=CANONICAL):
     return F, cpt[0], cpf[0]
 
 
-def rkck45_linear(lam, y0, h):
+def rkck45_linear(lam, y0, h, mode=CANONICAL):
     y = ctypes.c_double()
     e = ctypes.c_double()
-    ok = lib().bex_oracle_rkck45_linear(lam, y0, h, ctypes.byref(y), ctypes.byref(e))
+    ok = lib().bex_oracle_rkck45_linear(lam, y0, h, mode, ctypes.byref(y), ctypes.byref(e))
     return y.value, e.value, ok
 
 
-def rk4_linear(lam, y0, h):
+def rk4_linear(lam, y0, h, mode=CANONICAL):
     y = ctypes.c_double()
-    ok = lib().bex_oracle_rk4_linear(lam, y0, h, ctypes.byref(y))
+    ok = lib().bex_oracle_rk4_linear(lam, y0, h, mode, ctypes.byref(y))
     return y.value, ok
 
 

@@ -163,7 +163,8 @@ COUPLING = os.path.join(REPO, "tests", "golden", "CouplingMatrix_N32_D1_d150_RE2
 
 
 def app_grid(lo, hi, n):
-    """The driver's linear grid: lo + (i / (n - 1)) (hi - lo), the last point exactly hi."""
+    """The collapse-strength application's linear grid: lo + (i / (n - 1)) (hi - lo), the last
+    point exactly hi."""
     v = [lo + (i / (n - 1)) * (hi - lo) for i in range(n)]
     v[-1] = hi
     return np.array(v)

@@ -295,11 +295,15 @@ def test_bicg_iterates_match_textbook(oracle, mode, k):
 # Steppers
 
 
-def test_rkck45_orders(oracle):
+@pytest.mark.parametrize("literal", [False, True], ids=["canonical", "literal"])
+def test_rkck45_orders(oracle, literal):
+    # Both arithmetics: the canonical one forms every stage sum with fmas, the literal one with
+    # separate products and sums; each must be the Cash-Karp method.
+    mode = oracle.LITERAL if literal else oracle.CANONICAL
     lam = -1.3
     errs5, ests = [], []
     for h in (0.2, 0.1, 0.05):
-        y, e, ok = oracle.rkck45_linear(lam, 1.0, h)
+        y, e, ok = oracle.rkck45_linear(lam, 1.0, h, mode=mode)
         assert ok == 1
         errs5.append(abs(y - math.exp(lam * h)))
         ests.append(e - 1e-18)
@@ -317,20 +321,33 @@ def test_rkck45_orders(oracle):
     for s in range(6):
         k.append(lam * (1.0 + h * sum(A[s][j] * k[j] for j in range(s))))
     y4 = 1.0 + h * sum(b4[s] * k[s] for s in range(6))
-    y5, e, _ = oracle.rkck45_linear(lam, 1.0, h)
+    y5, e, _ = oracle.rkck45_linear(lam, 1.0, h, mode=mode)
     assert e - 1e-18 == pytest.approx(abs(y5 - y4), rel=1e-6)
 
 
-def test_rk4_order(oracle):
+def test_rkck45_arithmetics_agree_to_rounding(oracle):
+    # The canonical and literal steps differ only in rounding: a few ulp in the solution, and
+    # the error estimate (a difference of nearly equal sums) within its own cancellation.
+    for lam in (-1.3, 0.7, -25.0):
+        for h in (0.3, 0.05, 1e-3):
+            yc, ec, _ = oracle.rkck45_linear(lam, 1.0, h, mode=oracle.CANONICAL)
+            yl, el, _ = oracle.rkck45_linear(lam, 1.0, h, mode=oracle.LITERAL)
+            assert abs(yc - yl) <= 8 * np.spacing(abs(yl)), (lam, h, yc, yl)
+            assert abs(ec - el) <= 1e-12 * abs(lam) * h + 1e-6 * el, (lam, h, ec, el)
+
+
+@pytest.mark.parametrize("literal", [False, True], ids=["canonical", "literal"])
+def test_rk4_order(oracle, literal):
+    mode = oracle.LITERAL if literal else oracle.CANONICAL
     lam = 0.9
     errs = []
     for h in (0.2, 0.1):
-        y, ok = oracle.rk4_linear(lam, 1.0, h)
+        y, ok = oracle.rk4_linear(lam, 1.0, h, mode=mode)
         errs.append(abs(y - math.exp(lam * h)))
     assert 26 < errs[0] / errs[1] < 38  # O(h^5) local error
     # RK4 reproduces the degree-4 Taylor polynomial exactly for linear problems
     h = 0.3
-    y, _ = oracle.rk4_linear(lam, 1.0, h)
+    y, _ = oracle.rk4_linear(lam, 1.0, h, mode=mode)
     z = lam * h
     assert y == pytest.approx(1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24, rel=1e-15)
 
