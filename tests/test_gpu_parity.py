@@ -100,6 +100,28 @@ def test_config0_within_config1_batch(oracle):
     np.testing.assert_array_equal(s.stats()[sids[keep]], p.stats_table())
 
 
+def test_config1_full_size_layouts_and_reruns_agree():
+    """Size-independent properties at the full 16384-system size: the role-pair kernel (rows of
+    M in registers, DPP broadcasts, one system per wave) and the lane-group kernel (M in LDS, two
+    systems per wave) agree bit for bit in state, accessories, event counts and counters over a
+    fifth of a period, and a second role-pair launch repeats the first bit for bit."""
+    big = wl.config1()
+    print("case: config 1, 16384 systems x N=32, tau in [0, 0.2]: role pair twice, lane group "
+          "once", flush=True)
+    runs = []
+    for layout in (LAYOUT_ROLE_PAIR, LAYOUT_ROLE_PAIR, LAYOUT_GROUP):
+        s = run_gpu(big, periods=1, end_time=0.2, layout=layout)
+        runs.append([s.state().copy(), s.accessories().copy(), s.event_counts().copy(),
+                     s.stats().copy()])
+        s.close()
+    assert runs[0][3][:, 0].min() > 0  # every system stepped
+    for other in runs[1:]:
+        for a, b in zip(runs[0][:2], other[:2]):
+            np.testing.assert_array_equal(a.view(np.int64), b.view(np.int64))
+        for a, b in zip(runs[0][2:], other[2:]):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("N,layout,periods,end", [(32, LAYOUT_GROUP, 2, 1.0),
                                                    (32, LAYOUT_ROLE_PAIR, 2, 1.0),
                                                    (64, LAYOUT_ROLE_PAIR, 2, 0.3),
