@@ -1689,6 +1689,9 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
 
   int terminate = 0;
   int accepted = 0;
+  // Issue priority follows this system's accepted steps (raised below); a persistent wave that
+  // takes its next system from a queue starts it at the base level again.
+  __builtin_amdgcn_s_setprio(0);
   long long stage_count = 0;  // RK stages of this launch (sample selection)
   while (terminate == 0) {  // Solver.cuh:329-572
     int updateStep = 1, endReached = 0;
