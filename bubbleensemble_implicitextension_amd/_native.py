@@ -78,6 +78,8 @@ _SIGNATURES = {
     "bex_synchronise_device": ([ctypes.c_void_p], ctypes.c_int),
     "bex_insert_synchronisation_point": ([ctypes.c_void_p], ctypes.c_int),
     "bex_synchronise_solver": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_checkpoint_save": ([ctypes.c_void_p], ctypes.c_int),
+    "bex_checkpoint_restore": ([ctypes.c_void_p], ctypes.c_int),
     "bex_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "bex_reset_stats": ([ctypes.c_void_p], ctypes.c_int),
     "bex_timer_start": ([ctypes.c_void_p], ctypes.c_int),
@@ -85,6 +87,7 @@ _SIGNATURES = {
     "bex_enable_timeline": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "bex_get_timeline": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bex_peek_timeline": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "bex_set_warm_start_network": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "bex_enable_sampling": ([ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int], ctypes.c_int),
     "bex_get_samples": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_longlong,

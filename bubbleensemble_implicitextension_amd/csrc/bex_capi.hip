@@ -33,6 +33,7 @@ struct Buf {
 struct bex_solver {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t peek_stream = nullptr;  // bex_peek_timeline only (created on first use)
   hipEvent_t event = nullptr;
   hipEvent_t timer0 = nullptr, timer1 = nullptr;
   bex_problem_desc d{};
@@ -76,6 +77,7 @@ struct bex_solver {
   int DenseInterpolation = 0;         // 1: Hermite dense output at a uniform grid
   double* d_DenseAux = nullptr;       // [2 UD][TLT], allocated with the option
   bool SharedGlobalVariables = true, SharedCouplingMatrices = false;
+  unsigned char* d_ckpt = nullptr;    // device checkpoint (bex_checkpoint_save/restore)
   int config_epoch = 0;  // bumped by every option change (pools re-capture their graph)
   std::string err;
 };
@@ -594,9 +596,11 @@ int bex_destroy(bex_solver* s) {
   if (s->d_net) hipFree(s->d_net);
   if (s->d_samples) hipFree(s->d_samples);
   if (s->d_sample_count) hipFree(s->d_sample_count);
+  if (s->d_ckpt) hipFree(s->d_ckpt);
   if (s->event) hipEventDestroy(s->event);
   if (s->timer0) hipEventDestroy(s->timer0);
   if (s->timer1) hipEventDestroy(s->timer1);
+  if (s->peek_stream) hipStreamDestroy(s->peek_stream);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return BEX_OK;
@@ -822,6 +826,14 @@ int bex_set_host_dense_index(bex_solver* s, int sid, int var, double v) {
   if ((rc = bound(s, "SetHost", "SystemNumber", sid, 0, s->d.NS - 1))) return rc;
   if (var != BEX_DenseIndex)
     return fail(s, BEX_E_VARIABLE, "SetHost: this option needs different argument configuration or not applicable");
+  // The index counts stored points: 0 (empty) to NDO (full). Anything else would address
+  // outside the dense-output buffers.
+  if (!(v >= 0.0 && v <= (double)s->d.NDO)) {
+    char m[192];
+    snprintf(m, sizeof m, "SetHost: DenseIndex value %g out of range (acceptable values: 0-%d)", v,
+             s->d.NDO);
+    return fail(s, BEX_E_BOUND, m);
+  }
   static_cast<int*>(s->DenseOutputIndex.h)[sid] = (int)v;
   return BEX_OK;
 }
@@ -877,6 +889,21 @@ int bex_host_view(bex_solver* s, int var, void** ptr, long long* count, int* ele
   return BEX_OK;
 }
 
+// Host-side dense-output indices (also writable through host views) must lie in [0, NDO]
+// before they reach the kernel, which stores point idx of a system at offset idx.
+static int check_dense_index(bex_solver* s) {
+  const int* idx = static_cast<const int*>(s->DenseOutputIndex.h);
+  for (long long k = 0; k < s->DenseOutputIndex.n; k++)
+    if (idx[k] < 0 || idx[k] > s->d.NDO) {
+      char m[192];
+      snprintf(m, sizeof m,
+               "SynchroniseFromHostToDevice: DenseIndex of system %lld is %d (acceptable values: 0-%d)",
+               k, idx[k], s->d.NDO);
+      return fail(s, BEX_E_BOUND, m);
+    }
+  return BEX_OK;
+}
+
 int bex_sync_h2d(bex_solver* s, int var) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
@@ -884,6 +911,7 @@ int bex_sync_h2d(bex_solver* s, int var) {
   switch (var) {
     case BEX_CouplingMatrix: return upload_coupling(s);
     case BEX_DenseOutput:
+      if ((rc = check_dense_index(s))) return rc;
       if ((rc = copy_h2d(s, s->DenseOutputIndex))) return rc;
       if ((rc = copy_h2d(s, s->DenseOutputTimeInstances))) return rc;
       return copy_h2d(s, s->DenseOutputStates);
@@ -893,6 +921,7 @@ int bex_sync_h2d(bex_solver* s, int var) {
                     &s->UnitAccessories, &s->IntegerUnitAccessories, &s->SystemAccessories,
                     &s->IntegerSystemAccessories, &s->CouplingStrength, &s->CouplingIndex,
                     &s->DenseOutputIndex, &s->DenseOutputTimeInstances, &s->DenseOutputStates};
+      if ((rc = check_dense_index(s))) return rc;
       for (Buf* b : all)
         if ((rc = copy_h2d(s, *b))) return rc;
       return upload_coupling(s);
@@ -984,6 +1013,43 @@ int bex_synchronise_solver(bex_solver* s) {
   return BEX_OK;
 }
 
+// Buffers one Solve() writes back (Solver.cuh:601-648); each starts at a 256-byte offset of the
+// snapshot.
+static int checkpoint_copy(bex_solver* s, bool save) {
+  Buf* bufs[] = {&s->TimeDomain, &s->ActualState, &s->ActualTime, &s->UnitAccessories,
+                 &s->IntegerUnitAccessories, &s->SystemAccessories,
+                 &s->IntegerSystemAccessories, &s->DenseOutputIndex};
+  auto span = [](const Buf* b) { return b->n > 0 ? ((b->bytes() + 255) / 256) * 256 : (size_t)0; };
+  BEX_HIP(s, hipSetDevice(s->device));
+  if (!s->d_ckpt) {
+    if (!save) return fail(s, BEX_E_ARGUMENT, "bex_checkpoint_restore: no checkpoint was saved");
+    size_t total = 0;
+    for (const Buf* b : bufs) total += span(b);
+    BEX_HIP(s, hipMalloc((void**)&s->d_ckpt, total > 0 ? total : 256));
+  }
+  size_t off = 0;
+  for (Buf* b : bufs) {
+    if (b->n > 0) {
+      unsigned char* snap = s->d_ckpt + off;
+      const size_t bytes = (size_t)b->n * b->esz;
+      BEX_HIP(s, hipMemcpyAsync(save ? (void*)snap : b->d, save ? b->d : (const void*)snap, bytes,
+                                hipMemcpyDeviceToDevice, s->stream));
+    }
+    off += span(b);
+  }
+  return BEX_OK;
+}
+
+int bex_checkpoint_save(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  return checkpoint_copy(s, true);
+}
+
+int bex_checkpoint_restore(bex_solver* s) {
+  if (!s) return BEX_E_ARGUMENT;
+  return checkpoint_copy(s, false);
+}
+
 int bex_get_stats(bex_solver* s, long long* out) {
   if (!s || !out) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
@@ -1015,10 +1081,12 @@ int bex_enable_timeline(bex_solver* s, int on) {
   if (s) s->config_epoch++;
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
-  if (on && !s->d_timeline) {
-    BEX_HIP(s, hipMalloc((void**)&s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS));
+  if (on) {
+    // (Re)enabling clears the timeline, in stream order before the next launch.
+    if (!s->d_timeline)
+      BEX_HIP(s, hipMalloc((void**)&s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS));
     BEX_HIP(s, hipMemsetAsync(s->d_timeline, 0, sizeof(long long) * 2 * (size_t)s->d.NS, s->stream));
-  } else if (!on && s->d_timeline) {
+  } else if (s->d_timeline) {
     BEX_HIP(s, hipStreamSynchronize(s->stream));
     BEX_HIP(s, hipFree(s->d_timeline));
     s->d_timeline = nullptr;
@@ -1036,6 +1104,17 @@ int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second) {
   int khz = 0;
   BEX_HIP(s, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device));
   *ticks_per_second = 1e3 * khz;
+  return BEX_OK;
+}
+
+int bex_peek_timeline(bex_solver* s, long long* out) {
+  if (!s || !out) return BEX_E_ARGUMENT;
+  if (!s->d_timeline) return fail(s, BEX_E_ARGUMENT, "bex_peek_timeline: timeline not enabled");
+  BEX_HIP(s, hipSetDevice(s->device));
+  if (!s->peek_stream) BEX_HIP(s, hipStreamCreateWithFlags(&s->peek_stream, hipStreamNonBlocking));
+  BEX_HIP(s, hipMemcpyAsync(out, s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS,
+                            hipMemcpyDeviceToHost, s->peek_stream));
+  BEX_HIP(s, hipStreamSynchronize(s->peek_stream));
   return BEX_OK;
 }
 

@@ -1563,7 +1563,10 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     for (int i = 0; i < D::NSA; i++) S.sACC[i] = a.SystemAccessories[sid + i * NS];
     for (int i = 0; i < D::NiSA; i++) S.isACC[i] = a.IntegerSystemAccessories[sid + i * NS];
     S.UDT = 0;
-    S.DOIDX = (DENSE && a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
+    // The host keeps the index in [0, NDO]; anything outside is treated as a full buffer (no
+    // store), never used as an offset.
+    const int di = (DENSE && a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
+    S.DOIDX = (di < 0 || di > a.NDO) ? a.NDO : di;
   }
 
   // Unit scope into registers / private memory (Solver.cuh:234-257).

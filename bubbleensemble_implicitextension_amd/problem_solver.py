@@ -240,6 +240,17 @@ class ProblemSolver:
     def SynchroniseSolver(self):
         self._check(self._lib.bex_synchronise_solver(self._h))
 
+    # -- device checkpoint (no reference counterpart: the reference checkpoints through the
+    #    host with SynchroniseFromDeviceToHost(All), Interface.cuh:1310-1328) --------------
+    def checkpoint_save(self):
+        """Snapshot, in device memory, of everything a Solve() writes back (time domain, state,
+        time, accessories, dense-output index); asynchronous on the solver's stream."""
+        self._check(self._lib.bex_checkpoint_save(self._h))
+
+    def checkpoint_restore(self):
+        """Copies the last device snapshot back (asynchronous on the solver's stream)."""
+        self._check(self._lib.bex_checkpoint_restore(self._h))
+
     # -- counters ----------------------------------------------------------------------
     def stats(self):
         out = np.zeros(self.NS * _native.NSTAT, dtype=np.int64)
@@ -270,6 +281,14 @@ class ProblemSolver:
         if valid.any():
             t -= t[valid, 0].min()
         return t / tps.value
+
+    def finished_systems(self):
+        """Systems of the current (or last) launch that have finished, read while the launch
+        may still run (needs enable_timeline, which clears the count)."""
+        out = np.zeros(2 * self.NS, dtype=np.int64)
+        self._check(self._lib.bex_peek_timeline(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        return int(np.count_nonzero(out[1::2]))
 
     def reset_stats(self):
         self._check(self._lib.bex_reset_stats(self._h))

@@ -35,12 +35,11 @@ except ImportError:  # training and batch prediction then report that they need 
 from . import _native
 from . import collapse
 from . import workloads as wl
+from .batched import DEFAULT_NETWORK, LearnedBatchedSolver, cost_proxy  # noqa: F401
 from .problem_solver import MLP_NPARAM, SAMPLE_FLOATS
 from .problem_solver import ListOfSolverOptions as O
 from .problem_solver import ListOfVariables as V
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_NETWORK = os.path.join(HERE, "data", "warm_start_n32.npy")
 TRAIN_SEED = 7001  # held-out geometry/radius seed of the training ensemble
 
 
@@ -178,76 +177,6 @@ def guess_errors(rows, y):
     learned = np.abs(b * yl - x)[ok] / np.abs(x[ok])
     plain = np.abs(xu - x)[ok] / np.abs(x[ok])
     return float(np.median(learned)), float(np.median(plain))
-
-
-def cost_proxy(ens):
-    """Per-system cost estimate of the collapse workloads without running them: forcing
-    amplitude over frequency (a stronger, slower drive gives larger expansions, more violent
-    collapses and stiffer coupling solves)."""
-    return np.asarray(ens.pa_bar, dtype=np.float64) / np.asarray(ens.f_khz, dtype=np.float64)
-
-
-class LearnedBatchedSolver:
-    """Learned + batched: the `heavy` systems with the largest cost_proxy run with the learned
-    BiCG start, the others with the plain start, as two solver objects (each on its own stream)
-    launched together, the heavy ones first. The plain kernel keeps its occupancy for the bulk
-    (3 waves per SIMD against the learned kernel's 2) and pays no network evaluation where the
-    start saves nothing; the learned start shortens the systems that bound the launch.
-
-    Solve() is asynchronous like ProblemSolver.Solve(); synchronise() waits for both parts;
-    results() returns state, accessories, event counts and counters in the ensemble's order.
-    """
-
-    def __init__(self, ens, network, heavy=64, end_time=1.0):
-        if ens.NS < 2:
-            raise ValueError("a learned + batched split needs at least two systems")
-        order = np.argsort(-cost_proxy(ens), kind="stable")
-        heavy = max(1, min(int(heavy), ens.NS - 1))
-        self.heavy = np.sort(order[:heavy])
-        self.light = np.sort(order[heavy:])
-        self.NS, self.N = ens.NS, ens.N
-        self.parts = []
-        for sids, net in ((self.heavy, network), (self.light, None)):
-            sub = ens.subset(sids)
-            s = collapse.make_solver(sub)
-            collapse.fill(s, sub)
-            s.host_view(V.TimeDomain)[sub.NS:] = end_time
-            if net is not None:
-                s.set_warm_start_network(net)
-                s.SolverOption(O.WarmStart, 1)
-            s.SynchroniseFromHostToDevice(V.All)
-            self.parts.append((sids, s))
-
-    def Solve(self):
-        for _, s in self.parts:
-            s.Solve()
-
-    def synchronise(self):
-        for _, s in self.parts:
-            s.SynchroniseDevice()
-
-    def reset_stats(self):
-        for _, s in self.parts:
-            s.reset_stats()
-
-    def results(self):
-        out = {"state": np.zeros((self.NS, self.N, 2)), "acc": np.zeros((self.NS, self.N, wl.NUA)),
-               "events": np.zeros((self.NS, self.N), dtype=np.int64),
-               "stats": np.zeros((self.NS, _native.NSTAT), dtype=np.int64)}
-        for sids, s in self.parts:
-            s.SynchroniseFromDeviceToHost(V.All)
-            s.InsertSynchronisationPoint()  # the copies are asynchronous on the solver's stream
-            s.SynchroniseSolver()
-            out["state"][sids] = s.state()
-            out["acc"][sids] = s.accessories()
-            out["events"][sids] = s.event_counts()
-            out["stats"][sids] = s.stats()
-        return out
-
-    def close(self):
-        for _, s in self.parts:
-            s.close()
-        self.parts = []
 
 
 def train_network(samples=1_000_000, epochs=8, batch=4096, lr=2e-3, seed=0, systems=4096,

@@ -158,6 +158,19 @@ int bex_synchronise_device(bex_solver* s);
 int bex_insert_synchronisation_point(bex_solver* s);
 int bex_synchronise_solver(bex_solver* s);
 
+/* Device-resident checkpoint of everything one Solve() writes back (Solver.cuh:601-648):
+ * TimeDomain, ActualState, ActualTime, UnitAccessories, IntegerUnitAccessories,
+ * SystemAccessories, IntegerSystemAccessories and the dense-output index. bex_checkpoint_save
+ * copies those device buffers into a device snapshot (allocated by the first save),
+ * bex_checkpoint_restore copies the snapshot back; both are asynchronous device-to-device
+ * copies on the solver's stream, ordered with Solve() like any other call. The reference
+ * checkpoints the same variables through the host at a period boundary
+ * (SynchroniseFromDeviceToHost(All), Interface.cuh:1310-1328, then SetHost and
+ * SynchroniseFromHostToDevice); this keeps the copy in HBM. Restoring without a saved
+ * checkpoint fails with BEX_E_ARGUMENT. */
+int bex_checkpoint_save(bex_solver* s);
+int bex_checkpoint_restore(bex_solver* s);
+
 /* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);
  * copies device counters to `out` (synchronous). No reference counterpart. */
 int bex_get_stats(bex_solver* s, long long* out);
@@ -169,11 +182,15 @@ int bex_reset_stats(bex_solver* s);
 int bex_timer_start(bex_solver* s);
 int bex_timer_stop(bex_solver* s, double* elapsed_ms);
 
-/* Per-system timeline of the most recent launch: when enabled, the kernel stores the device
- * wall-clock counter at the start and end of each system ([NS][2]); bex_get_timeline copies
- * it and returns the counter frequency. No reference counterpart. */
+/* Per-system timeline of the most recent launch: when enabled (enabling also clears it), the
+ * kernel stores the device wall-clock counter at the start and end of each system ([NS][2]);
+ * bex_get_timeline copies it and returns the counter frequency. No reference counterpart. */
 int bex_enable_timeline(bex_solver* s, int on);
 int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second);
+/* The timeline as it stands while a launch may still run: copied on a stream of its own, so it
+ * does not wait for the solver's stream (an end tick of 0 marks a system not finished yet;
+ * enabling the timeline clears it). For progress reports of long launches. */
+int bex_peek_timeline(bex_solver* s, long long* out);
 
 /* BiCG warm start (no reference counterpart). params: BEX_MLP_NPARAM (4609) floats in the
  * layout of bubbleensemble_implicitextension_amd/csrc/bex_mlp.h (W1[6][64], W2[64][64], b2[64],

@@ -30,9 +30,23 @@
 
 #include "../bubbleensemble_implicitextension_amd/csrc/bex_detmath.h"
 
+// Planted mistakes, for the tests that show the oracle's pins reject them
+// (tests/test_oracle_event_pins.py builds variants of this file with one of them defined; the
+// oracle every other test uses is built with 0):
+//   1  event direction predicate mirrored (EventHandling.cuh:245-246, Solver.cuh:472-473)
+//   2  collapse slot off by one (SystemDefinition.cuh:87-90)
+//   3  dense-output skip count off by one (DenseOutput.cuh:385)
+#ifndef BEX_ORACLE_PLANT
+#define BEX_ORACLE_PLANT 0
+#endif
+
 namespace {
 
 constexpr double kPI = 3.14159265358979323846;  // SystemDefinition.cuh:4
+
+// Direction predicates of the event handling with the planted mirror (plant 1).
+inline bool dir_down(int dir) { return BEX_ORACLE_PLANT == 1 ? dir >= 0 : dir <= 0; }
+inline bool dir_up(int dir) { return BEX_ORACLE_PLANT == 1 ? dir <= 0 : dir >= 0; }
 
 struct MathCanonical {
   static double pow(double x, double y) { return bex::pow(x, y); }
@@ -93,9 +107,10 @@ inline void km_event(double* EF, const double* X) { EF[0] = X[1]; }  // SystemDe
 inline void km_action_after_event(int ncoll, const double* X, double* uACC, int* iuACC) {
   // SystemDefinition.cuh:85-91
   iuACC[0]++;
-  if (iuACC[0] < ncoll) {
-    uACC[iuACC[0]] = X[0];
-    uACC[iuACC[0] + ncoll] = X[0];
+  const int slot = iuACC[0] + (BEX_ORACLE_PLANT == 2 ? 1 : 0);
+  if (slot < ncoll) {
+    uACC[slot] = X[0];
+    uACC[slot + ncoll] = X[0];
   }
 }
 
@@ -622,8 +637,8 @@ void solve_system(const Ctx& c, int sid) {
           for (int i = 0; i < d.NE; i++) {
             double ea = EA[u * d.NE + i], en = EN[u * d.NE + i], tol = o.EventTolerance[i];
             int dir = o.EventDirection[i];
-            if (((ea > tol) && (en < -tol) && (dir <= 0)) ||
-                ((ea < -tol) && (en > tol) && (dir >= 0))) {
+            if (((ea > tol) && (en < -tol) && dir_down(dir)) ||
+                ((ea < -tol) && (en > tol) && dir_up(dir))) {
               double cand = -ea / (en - ea) * dt;
               if (cand < evStep) evStep = cand;
               corrected = 1;
@@ -661,8 +676,8 @@ void solve_system(const Ctx& c, int sid) {
           for (int i = 0; i < d.NE; i++) {
             double ea = EA[u * d.NE + i], en = EN[u * d.NE + i], tol = o.EventTolerance[i];
             int dir = o.EventDirection[i];
-            if (((ea > tol) && (std::fabs(en) < tol) && (dir <= 0)) ||
-                ((ea < -tol) && (std::fabs(en) < tol) && (dir >= 0))) {
+            if (((ea > tol) && (std::fabs(en) < tol) && dir_down(dir)) ||
+                ((ea < -tol) && (std::fabs(en) < tol) && dir_up(dir))) {
               km_action_after_event(ncoll, &X[u * UD], &A[u * d.NUA], &IA[u * d.NiUA]);
               st[BEX_STAT_EVENTS]++;
             }
@@ -683,7 +698,7 @@ void solve_system(const Ctx& c, int sid) {
     if (d.NDO > 0 && !interp) {
       if (updateStep == 1) {
         updateDO = ((doIdx < d.NDO) && (doTime < t) &&
-                    (skipped >= (o.DenseOutputSaveFrequency - 1)))
+                    (skipped >= (o.DenseOutputSaveFrequency - (BEX_ORACLE_PLANT == 3 ? 0 : 1))))
                        ? 1
                        : 0;
         if ((doIdx < d.NDO) && ((endReached == 1) || (udt == 1))) updateDO = 1;

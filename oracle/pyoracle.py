@@ -6,6 +6,7 @@ product path.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -73,7 +74,26 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_variant(plant, out_path):
+    """Builds the oracle with planted mistake `plant` (BEX_ORACLE_PLANT in bex_oracle.cpp) into
+    out_path, for tests that show a pin rejects that mistake."""
+    subprocess.run(["make", "-s", "-C", HERE, "variant", f"PLANT={int(plant)}",
+                    f"OUT={out_path}"], check=True)
+
+
 _lib = None
+
+
+@contextlib.contextmanager
+def using_library(path):
+    """Routes every oracle call inside the block to another build of the oracle."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def lib():
@@ -81,36 +101,40 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.bex_oracle_solve.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(Options),
-                                       ctypes.POINTER(Buffers),
-                                       ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
-                                       ctypes.c_int]
-        L.bex_oracle_solve.restype = ctypes.c_int
-        L.bex_oracle_bicg.argtypes = [ctypes.c_int] + [_DP] * 5 + [ctypes.c_double,
-                                                                   ctypes.c_double,
-                                                                   ctypes.c_int,
-                                                                   ctypes.c_int, _DP]
-        L.bex_oracle_bicg.restype = ctypes.c_int
-        L.bex_oracle_km_rhs.argtypes = [_DP, ctypes.c_double, _DP, ctypes.c_int, _DP, _DP, _DP]
-        L.bex_oracle_km_rhs.restype = None
-        L.bex_oracle_pow.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
-        L.bex_oracle_pow.restype = ctypes.c_double
-        L.bex_oracle_sincospi.argtypes = [ctypes.c_double, ctypes.c_int, _DP, _DP]
-        L.bex_oracle_sin.argtypes = [ctypes.c_double, ctypes.c_int]
-        L.bex_oracle_sin.restype = ctypes.c_double
-        L.bex_oracle_cos.argtypes = [ctypes.c_double, ctypes.c_int]
-        L.bex_oracle_cos.restype = ctypes.c_double
-        L.bex_oracle_rkck45_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP, _DP]
-        L.bex_oracle_rkck45_linear.restype = ctypes.c_int
-        L.bex_oracle_rk4_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP]
-        L.bex_oracle_rk4_linear.restype = ctypes.c_int
-        L.bex_oracle_control.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                         ctypes.c_double, ctypes.POINTER(Options), ctypes.c_int,
-                                         _DP, _IP, _IP, _IP]
-        L.bex_oracle_control.restype = None
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    L.bex_oracle_solve.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(Options),
+                                   ctypes.POINTER(Buffers),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
+                                   ctypes.c_int]
+    L.bex_oracle_solve.restype = ctypes.c_int
+    L.bex_oracle_bicg.argtypes = [ctypes.c_int] + [_DP] * 5 + [ctypes.c_double,
+                                                               ctypes.c_double,
+                                                               ctypes.c_int,
+                                                               ctypes.c_int, _DP]
+    L.bex_oracle_bicg.restype = ctypes.c_int
+    L.bex_oracle_km_rhs.argtypes = [_DP, ctypes.c_double, _DP, ctypes.c_int, _DP, _DP, _DP]
+    L.bex_oracle_km_rhs.restype = None
+    L.bex_oracle_pow.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.bex_oracle_pow.restype = ctypes.c_double
+    L.bex_oracle_sincospi.argtypes = [ctypes.c_double, ctypes.c_int, _DP, _DP]
+    L.bex_oracle_sin.argtypes = [ctypes.c_double, ctypes.c_int]
+    L.bex_oracle_sin.restype = ctypes.c_double
+    L.bex_oracle_cos.argtypes = [ctypes.c_double, ctypes.c_int]
+    L.bex_oracle_cos.restype = ctypes.c_double
+    L.bex_oracle_rkck45_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP, _DP]
+    L.bex_oracle_rkck45_linear.restype = ctypes.c_int
+    L.bex_oracle_rk4_linear.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _DP]
+    L.bex_oracle_rk4_linear.restype = ctypes.c_int
+    L.bex_oracle_control.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.POINTER(Options), ctypes.c_int,
+                                     _DP, _IP, _IP, _IP]
+    L.bex_oracle_control.restype = None
+    return L
 
 
 def _dp(a):
