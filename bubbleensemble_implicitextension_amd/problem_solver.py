@@ -317,6 +317,65 @@ class ProblemSolver:
                 ctypes.byref(n)))
         return out
 
+    # -- text dumps (Interface.cuh:1531-1947) ---------------------------------------------
+    def thread_layout(self):
+        d = self.dims
+        return ThreadLayout.make(d["NS"], d["UPS"], d["TPB"], d["SPB"])
+
+    def Print(self, var, serial=None, directory="."):
+        """Print(var): the host copy of `var` to <var>InSolverObject.txt; Print(CouplingMatrix,
+        k): the original coupling matrix k to CouplingMatrixInSolverObject_k.txt;
+        Print(DenseOutput, sid): system sid's report and stored time series to
+        DenseOutput_sid.txt. Same file names and text as the reference; files go to
+        `directory` (the working directory by default)."""
+        import os
+
+        var = ListOfVariables(var)
+        d = self.dims
+        t = self.thread_layout()
+        V = ListOfVariables
+        if serial is None:
+            path = os.path.join(directory, f"{var.name}InSolverObject.txt")
+            unit = {V.ActualState: d["UD"], V.UnitParameters: d["NUP"],
+                    V.UnitAccessories: d["NUA"], V.IntegerUnitAccessories: d["NiUA"]}
+            system = {V.TimeDomain: (d["NS"], 2), V.ActualTime: (d["NS"], 1),
+                      V.SystemParameters: (d["NS"], d["NSP"]),
+                      V.GlobalParameters: (d["NGP"], 1),
+                      V.IntegerGlobalParameters: (d["NiGP"], 1),
+                      V.SystemAccessories: (d["NS"], d["NSA"]),
+                      V.IntegerSystemAccessories: (d["NS"], d["NiSA"]),
+                      V.CouplingStrength: (d["NS"], d["NC"]), V.CouplingIndex: (d["NC"], 1)}
+            if var in unit:
+                write_unit_scope(path, unit[var], self.host_view(var), t)
+            elif var in system:
+                rows, cols = system[var]
+                write_system_scope(path, rows, cols, self.host_view(var))
+            else:
+                raise _native.BexError(-3, f"Print: option {var.name}: this option needs "
+                                           "different argument configuration or not applicable")
+            return path
+        k = int(serial)
+        if var == V.CouplingMatrix:
+            _print_bound("CouplingMatrix", k, d["NC"] - 1)
+            N = d["UPS"]
+            m = np.array([[self.GetHost(k, V.CouplingMatrix, r, c) for r in range(N)]
+                          for c in range(N)]).reshape(-1)  # column-major
+            path = os.path.join(directory, f"CouplingMatrixInSolverObject_{k}.txt")
+            write_system_scope(path, N, N, m)
+            return path
+        if var == V.DenseOutput:
+            _print_bound("DenseOutput", k, d["NS"] - 1)
+            path = os.path.join(directory, f"DenseOutput_{k}.txt")
+            write_dense_report(path, int(self.host_view(V.DenseIndex)[k]), k, t, d,
+                               {v: self.host_view(v) for v in (
+                                   V.UnitParameters, V.UnitAccessories, V.IntegerUnitAccessories,
+                                   V.SystemParameters, V.SystemAccessories,
+                                   V.IntegerSystemAccessories, V.GlobalParameters,
+                                   V.IntegerGlobalParameters, V.DenseTime, V.DenseState)})
+            return path
+        raise _native.BexError(-3, f"Print: option {var.name}: this option needs different "
+                                   "argument configuration or not applicable")
+
     # -- convenience views (reference layout with TPB == SPB*UPS) --------------------------
     def state(self):
         return self.host_view(ListOfVariables.ActualState).reshape(
@@ -329,6 +388,121 @@ class ProblemSolver:
     def event_counts(self):
         v = self.host_view(ListOfVariables.IntegerUnitAccessories)
         return v.reshape(-1, self.NS, self.UPS)[0] + 1
+
+
+# -- text dumps in the reference's format (Interface.cuh:1531-1805) ------------------------
+# Fields right-aligned in 18 characters, floating-point values in scientific notation with 10
+# digits (C++ iostream std::scientific, precision 10), each followed by a comma.
+
+class ThreadLayout:
+    """Logical-thread layout of the reference (Interface.cuh:386-391, 875-879)."""
+
+    def __init__(self, NS, UPS, SPB, ltpb, pad, tlt):
+        self.NS, self.UPS, self.SPB = NS, UPS, SPB
+        self.LogicalThreadsPerBlock, self.ThreadPaddingPerBlock = ltpb, pad
+        self.TotalLogicalThreads = tlt
+
+    @classmethod
+    def make(cls, NS, UPS, TPB, SPB):
+        ltpb = SPB * UPS
+        launches = ltpb // TPB + (1 if ltpb % TPB else 0)
+        pad = launches * TPB - ltpb
+        grid = NS // SPB + (1 if NS % SPB else 0)
+        return cls(NS, UPS, SPB, ltpb, pad, (ltpb + pad) * grid)
+
+    def thread_of(self, sid, uid):
+        return ((sid // self.SPB) * (self.LogicalThreadsPerBlock + self.ThreadPaddingPerBlock)
+                + (sid % self.SPB) * self.UPS + uid)
+
+
+def _field(v):
+    if isinstance(v, str):
+        return f"{v:>18}"
+    if isinstance(v, (int, np.integer)):
+        return f"{int(v):>18d}"
+    return f"{float(v):>18.10e}"
+
+
+def _fields(values):
+    return [_field(v) for v in values]
+
+
+def write_unit_scope(path, columns, data, t):
+    """One row per logical thread, one column per component (data[k * TLT + thread]); padded
+    threads and the unused threads of a partly filled last block print PADDING."""
+    rows = t.TotalLogicalThreads
+    per_block = t.LogicalThreadsPerBlock + t.ThreadPaddingPerBlock
+    limit = (t.NS // t.SPB) * per_block + (t.NS % t.SPB) * t.UPS
+    data = np.asarray(data)
+    cast = int if data.dtype.kind in "iu" else float
+    with open(path, "w") as f:
+        for i in range(rows):
+            real = (i % per_block) < t.LogicalThreadsPerBlock and i < limit
+            row = [(_field(cast(data[i + j * rows])) if real else _field("PADDING")) + ","
+                   for j in range(columns)]
+            f.write("".join(row) + "\n")
+
+
+def write_system_scope(path, rows, columns, data):
+    """rows x columns, column-major data[i + j * rows]."""
+    data = np.asarray(data)
+    cast = int if data.dtype.kind in "iu" else float
+    with open(path, "w") as f:
+        for i in range(rows):
+            f.write("".join(_field(cast(data[i + j * rows])) + "," for j in range(columns)) + "\n")
+
+
+def write_dense_report(path, points, sid, t, dims, bufs):
+    """Per-system report: unit tables, system and global values, then the stored time series
+    (one row per point: t, then U<j>/C<k> for every unit and component)."""
+    V = ListOfVariables
+    UPS, UD, tlt, NS = t.UPS, dims["UD"], t.TotalLogicalThreads, t.NS
+    out = []
+
+    def unit_table(title, count, data):
+        data = np.asarray(data)
+        cast = int if data.dtype.kind in "iu" else float
+        out.append(f"{title}:\n")
+        out.append(_field("Serial No.") + "," + ",".join(_field(f"U{u}") for u in range(UPS)) + "\n")
+        for k in range(count):
+            vals = [cast(data[t.thread_of(sid, u) + k * tlt]) for u in range(UPS)]
+            out.append(_field(k) + "," + ",".join(_fields(vals)) + "\n")
+        out.append("\n")
+
+    def row(title, count, data, first, stride):
+        data = np.asarray(data)
+        cast = int if data.dtype.kind in "iu" else float
+        out.append(f"{title}:\n")
+        out.append(",".join(_fields(cast(data[first + k * stride]) for k in range(count))))
+        out.append("\n\n")
+
+    unit_table("UnitParameters", dims["NUP"], bufs[V.UnitParameters])
+    unit_table("UnitAccessories", dims["NUA"], bufs[V.UnitAccessories])
+    unit_table("IntegerUnitAccessories", dims["NiUA"], bufs[V.IntegerUnitAccessories])
+    row("SystemParameters", dims["NSP"], bufs[V.SystemParameters], sid, NS)
+    row("SystemAccessories", dims["NSA"], bufs[V.SystemAccessories], sid, NS)
+    row("IntegerSystemAccessories", dims["NiSA"], bufs[V.IntegerSystemAccessories], sid, NS)
+    row("GlobalParameters", dims["NGP"], bufs[V.GlobalParameters], 0, 1)
+    row("IntegerGlobalParameters", dims["NiGP"], bufs[V.IntegerGlobalParameters], 0, 1)
+    out.append("Time series:\n")
+    out.append(_field("t") + "," + ",".join(_field(f"U{u}/C{c}") for u in range(UPS)
+                                            for c in range(UD)) + "\n")
+    times, states = np.asarray(bufs[V.DenseTime]), np.asarray(bufs[V.DenseState])
+    for i in range(points):
+        vals = [float(states[t.thread_of(sid, u) + c * tlt + i * tlt * UD])
+                for u in range(UPS) for c in range(UD)]
+        out.append(_field(float(times[sid + i * NS])) + "," + ",".join(_fields(vals)) + "\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def _print_bound(what, index, last):
+    """BoundCheck of Print (Interface.cuh:718-733)."""
+    if 0 <= index <= last:
+        return
+    acceptable = "none" if last < 0 else f"0-{last}"
+    raise _native.BexError(-2, f"Print: option {what}: acceptable index {acceptable}, current "
+                               f"index {index}")
 
 
 class ProblemPool:

@@ -101,6 +101,163 @@ inline std::string VariablesToString(ListOfVariables Option) {
   return (i >= 0 && i < n) ? names[i] : "Non-existent variable!";
 }
 
+// --- Text dumps of host buffers (Interface.cuh:1531-1805) ---------------------------------
+// One implementation, written in this header's own words, of the reference's three file
+// writers: fields right-aligned in 18 characters, floating-point values in scientific notation
+// with 10 digits, every field followed by a comma.
+
+namespace bex_detail {
+
+constexpr int kDumpWidth = 18;  // Interface.cuh:1540-1542
+
+inline void DumpFormat(std::ofstream& f) {
+  f.precision(10);
+  f.flags(std::ios::scientific);
+}
+
+// Reference layout of the logical threads (Interface.cuh:386-391, 875-879).
+struct ThreadLayout {
+  int NS, UPS, SPB, LogicalThreadsPerBlock, ThreadPaddingPerBlock;
+  long long TotalLogicalThreads;
+  long long ThreadOf(int sid, int uid) const {
+    return (long long)(sid / SPB) * (LogicalThreadsPerBlock + ThreadPaddingPerBlock) +
+           (long long)(sid % SPB) * UPS + uid;
+  }
+};
+
+inline ThreadLayout MakeThreadLayout(int NS, int UPS, int TPB, int SPB) {
+  ThreadLayout t{NS, UPS, SPB, SPB * UPS, 0, 0};
+  const int launches = t.LogicalThreadsPerBlock / TPB + (t.LogicalThreadsPerBlock % TPB ? 1 : 0);
+  t.ThreadPaddingPerBlock = launches * TPB - t.LogicalThreadsPerBlock;
+  const int grid = NS / SPB + (NS % SPB ? 1 : 0);
+  t.TotalLogicalThreads = (long long)(t.LogicalThreadsPerBlock + t.ThreadPaddingPerBlock) * grid;
+  return t;
+}
+
+// Unit scope: one row per logical thread, one column per component; padded threads and the
+// unused threads of a partly filled last block print PADDING (Interface.cuh:1531-1575).
+template <typename T>
+void WriteUnitScope(const std::string& name, int columns, const T* data, const ThreadLayout& t) {
+  std::ofstream f(name.c_str());
+  DumpFormat(f);
+  const long long rows = t.TotalLogicalThreads;
+  const long long perBlock = t.LogicalThreadsPerBlock + t.ThreadPaddingPerBlock;
+  const long long limit = (long long)(t.NS / t.SPB) * perBlock + (long long)(t.NS % t.SPB) * t.UPS;
+  for (long long i = 0; i < rows; i++) {
+    const bool real = (i % perBlock) < t.LogicalThreadsPerBlock && i < limit;
+    for (int j = 0; j < columns; j++) {
+      f.width(kDumpWidth);
+      if (real)
+        f << data[i + (long long)j * rows];
+      else
+        f << "PADDING";
+      f << ',';
+    }
+    f << '\n';
+  }
+}
+
+// System and global scope: rows x columns, column-major (Interface.cuh:1577-1601).
+template <typename T>
+void WriteSystemScope(const std::string& name, long long rows, int columns, const T* data) {
+  std::ofstream f(name.c_str());
+  DumpFormat(f);
+  for (long long i = 0; i < rows; i++) {
+    for (int j = 0; j < columns; j++) {
+      f.width(kDumpWidth);
+      f << data[i + (long long)j * rows] << ',';
+    }
+    f << '\n';
+  }
+}
+
+// Host buffers and dimensions the per-system dense-output report reads.
+struct DenseReport {
+  ThreadLayout t;
+  int UD, NUP, NUA, NiUA, NSP, NSA, NiSA, NGP, NiGP;
+  const double *UnitParameters, *UnitAccessories, *SystemParameters, *SystemAccessories,
+      *GlobalParameters, *TimeInstances, *States;
+  const int *IntegerUnitAccessories, *IntegerSystemAccessories, *IntegerGlobalParameters;
+};
+
+template <typename T>
+void ReportUnitTable(std::ofstream& f, const char* title, int count, const T* data,
+                     const ThreadLayout& t, int sid) {
+  f << title << ":\n";
+  f.width(kDumpWidth);
+  f << "Serial No." << ',';
+  for (int u = 0; u < t.UPS; u++) {
+    f.width(kDumpWidth);
+    f << "U" + std::to_string(u);
+    if (u < t.UPS - 1) f << ',';
+  }
+  f << '\n';
+  for (int k = 0; k < count; k++) {
+    f.width(kDumpWidth);
+    f << k << ',';
+    for (int u = 0; u < t.UPS; u++) {
+      f.width(kDumpWidth);
+      f << data[t.ThreadOf(sid, u) + (long long)k * t.TotalLogicalThreads];
+      if (u < t.UPS - 1) f << ',';
+    }
+    f << '\n';
+  }
+  f << '\n';
+}
+
+template <typename T>
+void ReportRow(std::ofstream& f, const char* title, int count, const T* data, long long first,
+               long long stride) {
+  f << title << ":\n";
+  for (int k = 0; k < count; k++) {
+    f.width(kDumpWidth);
+    f << data[first + (long long)k * stride];
+    if (k < count - 1) f << ',';
+  }
+  f << '\n' << '\n';
+}
+
+// Per-system report: its unit tables, system and global values, then its dense-output time
+// series, one row per stored point (Interface.cuh:1603-1805).
+inline void WriteDenseOutputReport(const std::string& name, int points, int sid,
+                                   const DenseReport& r) {
+  std::ofstream f(name.c_str());
+  DumpFormat(f);
+  const ThreadLayout& t = r.t;
+  ReportUnitTable(f, "UnitParameters", r.NUP, r.UnitParameters, t, sid);
+  ReportUnitTable(f, "UnitAccessories", r.NUA, r.UnitAccessories, t, sid);
+  ReportUnitTable(f, "IntegerUnitAccessories", r.NiUA, r.IntegerUnitAccessories, t, sid);
+  ReportRow(f, "SystemParameters", r.NSP, r.SystemParameters, sid, t.NS);
+  ReportRow(f, "SystemAccessories", r.NSA, r.SystemAccessories, sid, t.NS);
+  ReportRow(f, "IntegerSystemAccessories", r.NiSA, r.IntegerSystemAccessories, sid, t.NS);
+  ReportRow(f, "GlobalParameters", r.NGP, r.GlobalParameters, 0, 1);
+  ReportRow(f, "IntegerGlobalParameters", r.NiGP, r.IntegerGlobalParameters, 0, 1);
+  f << "Time series:\n";
+  f.width(kDumpWidth);
+  f << "t" << ',';
+  for (int u = 0; u < t.UPS; u++)
+    for (int c = 0; c < r.UD; c++) {
+      f.width(kDumpWidth);
+      f << "U" + std::to_string(u) + "/C" + std::to_string(c);
+      if (u < t.UPS - 1 || c < r.UD - 1) f << ',';
+    }
+  f << '\n';
+  const long long stateSize = t.TotalLogicalThreads * r.UD;
+  for (int i = 0; i < points; i++) {
+    f.width(kDumpWidth);
+    f << r.TimeInstances[sid + (long long)i * t.NS] << ',';
+    for (int u = 0; u < t.UPS; u++)
+      for (int c = 0; c < r.UD; c++) {
+        f.width(kDumpWidth);
+        f << r.States[t.ThreadOf(sid, u) + (long long)c * t.TotalLogicalThreads + i * stateSize];
+        if (u < t.UPS - 1 || c < r.UD - 1) f << ',';
+      }
+    f << '\n';
+  }
+}
+
+}  // namespace bex_detail
+
 // --- Device helpers (Interface.cuh:73-75, 259-345) --------------------------------------
 // Same names and output as the reference's free functions, for the HIP devices. The
 // "compute capability" is the architecture revision HIP reports (9.5 for gfx950), followed by
@@ -197,25 +354,27 @@ class ProblemSolver {
     std::exit(EXIT_FAILURE);
   }
 
-  void WriteToFile(const std::string& name, ListOfVariables var, int columns) {
+  // Host copy of a variable (the pinned buffer the Set/Get calls and synchronisation use).
+  template <typename T>
+  const T* HostBuffer(ListOfVariables var) {
     void* ptr = nullptr;
     long long count = 0;
     int esz = 8;
     Check(bex_host_view(handle_, var, &ptr, &count, &esz), "Print");
-    std::ofstream f(name.c_str());
-    f.precision(14);
-    f.flags(std::ios::scientific);
-    const long long rows = columns > 0 ? count / columns : 0;
-    for (long long r = 0; r < rows; r++) {
-      for (int c = 0; c < columns; c++) {
-        long long i = r + (long long)c * rows;
-        if (esz == 8)
-          f << std::setw(23) << static_cast<double*>(ptr)[i] << ' ';
-        else
-          f << std::setw(8) << static_cast<int*>(ptr)[i] << ' ';
-      }
-      f << '\n';
-    }
+    return static_cast<const T*>(ptr);
+  }
+
+  static bex_detail::ThreadLayout Layout() { return bex_detail::MakeThreadLayout(NS, UPS, TPB, SPB); }
+
+  // The original (uncompacted) coupling matrix k as GetHost returns it (Interface.cuh:1076).
+  std::vector<double> OriginalCoupling(int k) {
+    std::vector<double> m((size_t)UPS * UPS);
+    for (int col = 0; col < UPS; col++)
+      for (int row = 0; row < UPS; row++)
+        Check(bex_get_host_coupling(handle_, k, CouplingMatrix, row, col,
+                                    &m[row + (size_t)col * UPS]),
+              "Print");
+    return m;
   }
 
  public:
@@ -319,28 +478,96 @@ class ProblemSolver {
     return (T)v;
   }
 
-  // Interface.cuh:1531-1947: host buffers to text files, one row per logical thread (unit
-  // scope) or per system (system scope), one column per component.
+  // Interface.cuh:1807-1920: the host copy of a variable to <Variable>InSolverObject.txt in the
+  // working directory (unit scope: one row per logical thread; system scope: one row per
+  // system; global scope: one row per value).
   void Print(ListOfVariables var) {
-    int cols = 1;
+    const bex_detail::ThreadLayout t = Layout();
+    const std::string file = VariablesToString(var) + "InSolverObject.txt";
     switch (var) {
-      case ActualState: cols = UD; break;
-      case UnitParameters: cols = NUP; break;
-      case UnitAccessories: cols = NUA; break;
-      case IntegerUnitAccessories: cols = NiUA; break;
-      case TimeDomain: cols = 2; break;
-      case SystemParameters: cols = NSP; break;
-      case SystemAccessories: cols = NSA; break;
-      case IntegerSystemAccessories: cols = NiSA; break;
-      case CouplingStrength: cols = NC; break;
-      default: cols = 1; break;
+      case TimeDomain: bex_detail::WriteSystemScope(file, NS, 2, HostBuffer<double>(var)); break;
+      case ActualTime: bex_detail::WriteSystemScope(file, NS, 1, HostBuffer<double>(var)); break;
+      case ActualState: bex_detail::WriteUnitScope(file, UD, HostBuffer<double>(var), t); break;
+      case UnitParameters: bex_detail::WriteUnitScope(file, NUP, HostBuffer<double>(var), t); break;
+      case SystemParameters:
+        bex_detail::WriteSystemScope(file, NS, NSP, HostBuffer<double>(var));
+        break;
+      case GlobalParameters:
+        bex_detail::WriteSystemScope(file, NGP, 1, HostBuffer<double>(var));
+        break;
+      case IntegerGlobalParameters:
+        bex_detail::WriteSystemScope(file, NiGP, 1, HostBuffer<int>(var));
+        break;
+      case UnitAccessories: bex_detail::WriteUnitScope(file, NUA, HostBuffer<double>(var), t); break;
+      case IntegerUnitAccessories:
+        bex_detail::WriteUnitScope(file, NiUA, HostBuffer<int>(var), t);
+        break;
+      case SystemAccessories:
+        bex_detail::WriteSystemScope(file, NS, NSA, HostBuffer<double>(var));
+        break;
+      case IntegerSystemAccessories:
+        bex_detail::WriteSystemScope(file, NS, NiSA, HostBuffer<int>(var));
+        break;
+      case CouplingStrength:
+        bex_detail::WriteSystemScope(file, NS, NC, HostBuffer<double>(var));
+        break;
+      case CouplingIndex: bex_detail::WriteSystemScope(file, NC, 1, HostBuffer<int>(var)); break;
+      default: PrintOptionError(var);
     }
-    WriteToFile(std::string(VariableName(var)) + ".txt", var, cols);
-  }
-  void Print(ListOfVariables var, int serial) {
-    WriteToFile(std::string(VariableName(var)) + "_" + std::to_string(serial) + ".txt", var, 1);
   }
 
+  // Interface.cuh:1922-1947: Print(CouplingMatrix, k) writes the original matrix k to
+  // CouplingMatrixInSolverObject_k.txt; Print(DenseOutput, sid) writes system sid's report and
+  // stored time series to DenseOutput_sid.txt. Both check the index first.
+  void Print(ListOfVariables var, int serial) {
+    if (var == CouplingMatrix) {
+      PrintBoundCheck("CouplingMatrix", serial, NC - 1);
+      const std::vector<double> m = OriginalCoupling(serial);
+      bex_detail::WriteSystemScope("CouplingMatrixInSolverObject_" + std::to_string(serial) + ".txt",
+                                   UPS, UPS, m.data());
+    } else if (var == DenseOutput) {
+      PrintBoundCheck("DenseOutput", serial, NS - 1);
+      bex_detail::DenseReport r{Layout(), UD, NUP, NUA, NiUA, NSP, NSA, NiSA, NGP, NiGP,
+                                HostBuffer<double>(UnitParameters),
+                                HostBuffer<double>(UnitAccessories),
+                                HostBuffer<double>(SystemParameters),
+                                HostBuffer<double>(SystemAccessories),
+                                HostBuffer<double>(GlobalParameters),
+                                HostBuffer<double>(DenseTime),
+                                HostBuffer<double>(DenseState),
+                                HostBuffer<int>(IntegerUnitAccessories),
+                                HostBuffer<int>(IntegerSystemAccessories),
+                                HostBuffer<int>(IntegerGlobalParameters)};
+      const int points = HostBuffer<int>(DenseIndex)[serial];
+      bex_detail::WriteDenseOutputReport("DenseOutput_" + std::to_string(serial) + ".txt", points,
+                                         serial, r);
+    } else {
+      PrintOptionError(var);
+    }
+  }
+
+ private:
+  // Interface.cuh:718-733 (BoundCheck) and :1942-1945.
+  static void PrintBoundCheck(const char* what, int index, int last) {
+    if (index >= 0 && index <= last) return;
+    std::cerr << "ERROR: In solver member function Print!" << std::endl;
+    std::cerr << "       Option: " << what << std::endl;
+    if (last < 0)
+      std::cerr << "       Acceptable index: none" << std::endl;
+    else
+      std::cerr << "       Acceptable index: 0-" << last << std::endl;
+    std::cerr << "       Current index:    " << index << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+  static void PrintOptionError(ListOfVariables var) {
+    std::cerr << "ERROR: In solver member function Print!" << std::endl;
+    std::cerr << "       Option: " << VariablesToString(var) << std::endl;
+    std::cerr << "       This option needs different argument configuration or not applicable!"
+              << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+
+ public:
   // Interface.cuh:1949-1983
   void Solve() { Check(bex_solve(handle_), "Solve"); }
   void SynchroniseDevice() { Check(bex_synchronise_device(handle_), "SynchroniseDevice"); }
@@ -358,28 +585,10 @@ class ProblemSolver {
     Check(bex_set_warm_start_network(handle_, params), "SetWarmStartNetwork");
   }
 
-  static const char* VariableName(ListOfVariables v) {
-    static const char* names[] = {"All",
-                                  "TimeDomain",
-                                  "ActualTime",
-                                  "ActualState",
-                                  "UnitParameters",
-                                  "SystemParameters",
-                                  "GlobalParameters",
-                                  "IntegerGlobalParameters",
-                                  "UnitAccessories",
-                                  "IntegerUnitAccessories",
-                                  "SystemAccessories",
-                                  "IntegerSystemAccessories",
-                                  "CouplingMatrix",
-                                  "CouplingStrength",
-                                  "CouplingIndex",
-                                  "DenseOutput",
-                                  "DenseIndex",
-                                  "DenseTime",
-                                  "DenseState"};
-    return names[(int)v];
-  }
+  // Device checkpoint of everything a Solve() writes back (no reference counterpart; the
+  // reference checkpoints through the host, Interface.cuh:1310-1328).
+  void CheckpointSave() { Check(bex_checkpoint_save(handle_), "CheckpointSave"); }
+  void CheckpointRestore() { Check(bex_checkpoint_restore(handle_), "CheckpointRestore"); }
 };
 
 // Uneven ensembles (no reference counterpart): size classes of systems with their own

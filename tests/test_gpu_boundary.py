@@ -213,3 +213,114 @@ def test_cpp_driver_matches_python_protocol(tmp_path):
     np.testing.assert_array_equal(a[:, :3], b[:, :3])  # system, PA, f
     np.testing.assert_allclose(a[:, 3:], b[:, 3:], rtol=2e-3, atol=0)
     assert (a[:, 3:] > 0).mean() > 0.5  # collapses were recorded
+
+
+PRINT_APP = r'''
+#include "CoupledSystems_PerBlock_Interface.hpp"
+// 3 systems of 32 units, two systems per block of 48 threads: 64 logical threads per block,
+// padded to 96; the second block holds one system (Interface.cuh:386-391).
+typedef ProblemSolver<3, 32, 2, 48, 2, 1, 0, 0, 21, 0, 0, 0, 64, 1, 0, 0, 1, 4, RKCK45, double> S;
+int main() {
+  S s(0);
+  for (int sid = 0; sid < 3; sid++) {
+    s.SetHost(sid, TimeDomain, 0, 0.0);
+    s.SetHost(sid, TimeDomain, 1, 1.0 + sid);
+    s.SetHost(sid, ActualTime, 0, 0.125 * sid);
+    s.SetHost(sid, CouplingStrength, 0, 2.0 - sid);
+    s.SetHost(sid, DenseIndex, sid + 1);
+    for (int uid = 0; uid < 32; uid++) {
+      s.SetHost(sid, uid, ActualState, 0, 1.0 + 0.01 * sid + 0.001 * uid);
+      s.SetHost(sid, uid, ActualState, 1, -0.5 * uid);
+      for (int k = 0; k < 21; k++) s.SetHost(sid, uid, UnitParameters, k, 1e-3 * (k + 1) * (uid + 1) - sid);
+      for (int k = 0; k < 64; k++) s.SetHost(sid, uid, UnitAccessories, k, k == uid ? 3.5 : 0.0);
+      s.SetHost(sid, uid, IntegerUnitAccessories, 0, uid - sid);
+      for (int i = 0; i < 4; i++)
+        for (int c = 0; c < 2; c++) s.SetHost(sid, uid, DenseState, c, i, 100.0 * i + 10.0 * c + uid);
+    }
+    for (int i = 0; i < 4; i++) s.SetHost(sid, DenseTime, i, 0.25 * i + sid);
+  }
+  s.SetHost(CouplingIndex, 0, 0);
+  for (int r = 0; r < 32; r++)
+    for (int c = 0; c < 32; c++) s.SetHost(0, CouplingMatrix, r, c, r == c ? 0.0 : 1.0 / (1 + r + 2 * c));
+  ListOfVariables vars[] = {TimeDomain, ActualTime, ActualState, UnitParameters, UnitAccessories,
+                            IntegerUnitAccessories, CouplingStrength, CouplingIndex};
+  for (ListOfVariables v : vars) s.Print(v);
+  s.Print(CouplingMatrix, 0);
+  s.Print(DenseOutput, 1);
+  s.Print(DenseOutput, 2);
+  return 0;
+}
+'''
+
+
+def test_print_files_match_between_cpp_header_and_python(tmp_path):
+    """Print(var), Print(CouplingMatrix, k) and Print(DenseOutput, sid) give the same bytes from
+    the C++ header and from the Python mirror, on a padded layout (two systems per block of 48
+    threads); the format itself is pinned on the CPU (test_text_dump.py)."""
+    import shutil
+    from bubbleensemble_implicitextension_amd import _native
+
+    if not shutil.which("g++"):
+        pytest.skip("no host compiler")
+    print("case: Print on 3 systems x 32 units, SPB 2, TPB 48", flush=True)
+    cpp_dir, py_dir = tmp_path / "cpp", tmp_path / "py"
+    cpp_dir.mkdir()
+    py_dir.mkdir()
+    src = tmp_path / "print_app.cpp"
+    src.write_text(PRINT_APP)
+    exe = tmp_path / "print_app"
+    pkg = os.path.dirname(_native.LIB_PATH)
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(repo, "include"), str(src),
+                    "-L" + pkg, "-lbex", "-Wl,-rpath," + pkg, "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], cwd=cpp_dir, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    s = ps.ProblemSolver(3, 32, TPB=48, SPB=2, NDO=4)
+    for sid in range(3):
+        s.SetHost(sid, V.TimeDomain, 0, 0.0)
+        s.SetHost(sid, V.TimeDomain, 1, 1.0 + sid)
+        s.SetHost(sid, V.ActualTime, 0, 0.125 * sid)
+        s.SetHost(sid, V.CouplingStrength, 0, 2.0 - sid)
+        s.SetHost(sid, V.DenseIndex, sid + 1)
+        for uid in range(32):
+            s.SetHost(sid, uid, V.ActualState, 0, 1.0 + 0.01 * sid + 0.001 * uid)
+            s.SetHost(sid, uid, V.ActualState, 1, -0.5 * uid)
+            for k in range(21):
+                s.SetHost(sid, uid, V.UnitParameters, k, 1e-3 * (k + 1) * (uid + 1) - sid)
+            for k in range(64):
+                s.SetHost(sid, uid, V.UnitAccessories, k, 3.5 if k == uid else 0.0)
+            s.SetHost(sid, uid, V.IntegerUnitAccessories, 0, uid - sid)
+            for i in range(4):
+                for c in range(2):
+                    s.SetHost(sid, uid, V.DenseState, c, i, 100.0 * i + 10.0 * c + uid)
+        for i in range(4):
+            s.SetHost(sid, V.DenseTime, i, 0.25 * i + sid)
+    s.SetHost(V.CouplingIndex, 0, 0)
+    for row in range(32):
+        for col in range(32):
+            s.SetHost(0, V.CouplingMatrix, row, col,
+                      0.0 if row == col else 1.0 / (1 + row + 2 * col))
+    for v in (V.TimeDomain, V.ActualTime, V.ActualState, V.UnitParameters, V.UnitAccessories,
+              V.IntegerUnitAccessories, V.CouplingStrength, V.CouplingIndex):
+        s.Print(v, directory=str(py_dir))
+    s.Print(V.CouplingMatrix, 0, directory=str(py_dir))
+    s.Print(V.DenseOutput, 1, directory=str(py_dir))
+    s.Print(V.DenseOutput, 2, directory=str(py_dir))
+    with pytest.raises(BexError):
+        s.Print(V.CouplingMatrix, 1)  # NC = 1
+    with pytest.raises(BexError):
+        s.Print(V.DenseOutput, 3)
+    s.close()
+
+    names = sorted(os.listdir(cpp_dir))
+    assert names == sorted(os.listdir(py_dir))
+    assert "ActualStateInSolverObject.txt" in names and "DenseOutput_2.txt" in names
+    assert "CouplingMatrixInSolverObject_0.txt" in names
+    for n in names:
+        assert (cpp_dir / n).read_text() == (py_dir / n).read_text(), n
+    state = (cpp_dir / "ActualStateInSolverObject.txt").read_text().splitlines()
+    assert len(state) == 192
+    assert state[64].split(",")[0].strip() == "PADDING"  # the first block's padded threads
+    assert state[128].split(",")[0].strip() == "PADDING"  # the second block's empty system
+    assert state[96].split(",")[0].strip() == "1.0200000000e+00"  # system 2, unit 0
