@@ -396,6 +396,50 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True):
     return out
 
 
+def run_second_period_leg():
+    """Headline workload's second period (continuing from the end of the first, no dense
+    output): per-system finish times from the kernel timeline, so the launch's tail (the systems
+    near the Keller-Miksis Mach-1 singularity) and its bulk are reported apart."""
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    ens = wl.config1()
+    s = make_headline_solver(ens, 1.0, dense=False)
+    s.Solve()
+    s.SynchroniseDevice()
+    first = s.stats()
+    s.reset_stats()
+    s.enable_timeline(True)
+    s.SynchroniseDevice()
+    t0 = time.perf_counter()
+    s.Solve()
+    total, last = ens.NS, time.perf_counter()
+    while True:
+        done = s.finished_systems()
+        if done >= total:
+            break
+        if time.perf_counter() - last >= 20.0:
+            log(f"second period: {done} of {total} systems finished")
+            last = time.perf_counter()
+        time.sleep(0.05)
+    s.SynchroniseDevice()
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    end = s.timeline()[:, 1]
+    s.close()
+    order = np.argsort(-end)
+    return {"workload": "16384 systems x N=32, second period (continuing from the first)",
+            "wall_s": wall, "finish_s_quantiles": {q: float(np.quantile(end, float(q)))
+                                                   for q in ("0.5", "0.9", "0.99", "0.999")},
+            "last_systems": [{"system": int(k), "finish_s": float(end[k]),
+                              "accepted": int(st[k, 0]), "rejected": int(st[k, 1]),
+                              "forced_accepts": int(st[k, 7]),
+                              "iterations_per_rhs": iters_per_rhs(st, [k])} for k in order[:8]],
+            "accepted_per_system_first_period_mean": float(first[:, 0].mean()),
+            "accepted_per_system_second_period_mean": float(st[:, 0].mean()),
+            "accepted_per_system_second_period_median": float(np.median(st[:, 0])),
+            "fm1_flops_ratio_second_to_first": flops_fm1(st, ens.N) / flops_fm1(first, ens.N)}
+
+
 def run_config4_leg(end_time, ns=4096):
     """Config 4 (ns systems of N = 256 in a 1 mm sphere, per-system coupling) over
     [0, end_time] from rest, one persistent launch through a one-class pool."""
@@ -455,7 +499,7 @@ def main():
                     help="headline leg only (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration) and exit")
-    ap.add_argument("--leg", choices=("pool", "n256"),
+    ap.add_argument("--leg", choices=("pool", "n256", "period2"),
                     help="run only this leg (with --pool-end-time / --config4-end-time) and "
                          "print its record instead of the benchmark line")
     args = ap.parse_args()
@@ -464,7 +508,8 @@ def main():
     if args.leg:
         log(f"{args.leg} leg alone")
         rec = (run_pool_leg(args.pool_end_time, sequential=False) if args.leg == "pool"
-               else run_config4_leg(args.config4_end_time))
+               else run_config4_leg(args.config4_end_time) if args.leg == "n256"
+               else run_second_period_leg())
         print(json.dumps({"leg": args.leg, **rec}), flush=True)
         return 0
     if args.gpus != 1:

@@ -33,7 +33,6 @@ struct Buf {
 struct bex_solver {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t peek_stream = nullptr;  // bex_peek_timeline only (created on first use)
   hipEvent_t event = nullptr;
   hipEvent_t timer0 = nullptr, timer1 = nullptr;
   bex_problem_desc d{};
@@ -59,7 +58,11 @@ struct bex_solver {
   double* d_tol = nullptr;
   int* d_dir = nullptr;
   long long* d_stats = nullptr;
-  long long* d_timeline = nullptr;  // [NS][2] start/end wall-clock ticks, when enabled
+  // [NS][2] start/end wall-clock ticks, when enabled: pinned host memory mapped into the
+  // device's address space (the kernel stores to it over the bus, the host reads it without
+  // any GPU command, also while a launch runs); d_timeline is its device address.
+  long long* h_timeline = nullptr;
+  long long* d_timeline = nullptr;
 
   double InitialTimeStep = 1e-2, MaximumTimeStep = 1.0e6, MinimumTimeStep = 1.0e-12;
   double TimeStepGrowLimit = 5.0, TimeStepShrinkLimit = 0.1, DenseOutputMinimumTimeStep = 0.0;
@@ -591,7 +594,7 @@ int bex_destroy(bex_solver* s) {
   if (s->d_tol) hipFree(s->d_tol);
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
-  if (s->d_timeline) hipFree(s->d_timeline);
+  if (s->h_timeline) hipHostFree(s->h_timeline);
   if (s->d_DenseAux) hipFree(s->d_DenseAux);
   if (s->d_net) hipFree(s->d_net);
   if (s->d_samples) hipFree(s->d_samples);
@@ -600,7 +603,6 @@ int bex_destroy(bex_solver* s) {
   if (s->event) hipEventDestroy(s->event);
   if (s->timer0) hipEventDestroy(s->timer0);
   if (s->timer1) hipEventDestroy(s->timer1);
-  if (s->peek_stream) hipStreamDestroy(s->peek_stream);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return BEX_OK;
@@ -1078,17 +1080,23 @@ int bex_timer_stop(bex_solver* s, double* elapsed_ms) {
 }
 
 int bex_enable_timeline(bex_solver* s, int on) {
-  if (s) s->config_epoch++;
   if (!s) return BEX_E_ARGUMENT;
+  s->config_epoch++;
   BEX_HIP(s, hipSetDevice(s->device));
+  // No launch may still write the timeline while it is cleared or freed (a pool's graph runs
+  // on the pool's streams, hence the device-wide wait).
+  BEX_HIP(s, hipDeviceSynchronize());
+  const size_t bytes = sizeof(long long) * 2 * (size_t)s->d.NS;
   if (on) {
-    // (Re)enabling clears the timeline, in stream order before the next launch.
-    if (!s->d_timeline)
-      BEX_HIP(s, hipMalloc((void**)&s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS));
-    BEX_HIP(s, hipMemsetAsync(s->d_timeline, 0, sizeof(long long) * 2 * (size_t)s->d.NS, s->stream));
-  } else if (s->d_timeline) {
-    BEX_HIP(s, hipStreamSynchronize(s->stream));
-    BEX_HIP(s, hipFree(s->d_timeline));
+    if (!s->h_timeline) {
+      BEX_HIP(s, hipHostMalloc((void**)&s->h_timeline, bytes,
+                               hipHostMallocMapped | hipHostMallocCoherent));
+      BEX_HIP(s, hipHostGetDevicePointer((void**)&s->d_timeline, s->h_timeline, 0));
+    }
+    std::memset(s->h_timeline, 0, bytes);  // (re)enabling clears it
+  } else if (s->h_timeline) {
+    BEX_HIP(s, hipHostFree(s->h_timeline));
+    s->h_timeline = nullptr;
     s->d_timeline = nullptr;
   }
   return BEX_OK;
@@ -1096,11 +1104,10 @@ int bex_enable_timeline(bex_solver* s, int on) {
 
 int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second) {
   if (!s || !out || !ticks_per_second) return BEX_E_ARGUMENT;
-  if (!s->d_timeline) return fail(s, BEX_E_ARGUMENT, "bex_get_timeline: timeline not enabled");
+  if (!s->h_timeline) return fail(s, BEX_E_ARGUMENT, "bex_get_timeline: timeline not enabled");
   BEX_HIP(s, hipSetDevice(s->device));
-  BEX_HIP(s, hipMemcpyAsync(out, s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS,
-                            hipMemcpyDeviceToHost, s->stream));
-  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));  // pool launches: the class stream waits for them
+  std::memcpy(out, s->h_timeline, sizeof(long long) * 2 * (size_t)s->d.NS);
   int khz = 0;
   BEX_HIP(s, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device));
   *ticks_per_second = 1e3 * khz;
@@ -1109,12 +1116,10 @@ int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second) {
 
 int bex_peek_timeline(bex_solver* s, long long* out) {
   if (!s || !out) return BEX_E_ARGUMENT;
-  if (!s->d_timeline) return fail(s, BEX_E_ARGUMENT, "bex_peek_timeline: timeline not enabled");
-  BEX_HIP(s, hipSetDevice(s->device));
-  if (!s->peek_stream) BEX_HIP(s, hipStreamCreateWithFlags(&s->peek_stream, hipStreamNonBlocking));
-  BEX_HIP(s, hipMemcpyAsync(out, s->d_timeline, sizeof(long long) * 2 * (size_t)s->d.NS,
-                            hipMemcpyDeviceToHost, s->peek_stream));
-  BEX_HIP(s, hipStreamSynchronize(s->peek_stream));
+  if (!s->h_timeline) return fail(s, BEX_E_ARGUMENT, "bex_peek_timeline: timeline not enabled");
+  // Plain reads of the mapped host buffer: no stream, no wait for the running launch.
+  const volatile long long* src = s->h_timeline;
+  for (long long k = 0; k < 2LL * s->d.NS; k++) out[k] = src[k];
   return BEX_OK;
 }
 

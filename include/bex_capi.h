@@ -182,14 +182,15 @@ int bex_reset_stats(bex_solver* s);
 int bex_timer_start(bex_solver* s);
 int bex_timer_stop(bex_solver* s, double* elapsed_ms);
 
-/* Per-system timeline of the most recent launch: when enabled (enabling also clears it), the
- * kernel stores the device wall-clock counter at the start and end of each system ([NS][2]);
- * bex_get_timeline copies it and returns the counter frequency. No reference counterpart. */
+/* Per-system timeline of the most recent launch: when enabled, the kernel stores the device
+ * wall-clock counter at the start and end of each system ([NS][2]); bex_get_timeline waits for
+ * the solver's stream, copies it and returns the counter frequency. Enabling waits for the
+ * whole device, then clears the timeline. No reference counterpart. */
 int bex_enable_timeline(bex_solver* s, int on);
 int bex_get_timeline(bex_solver* s, long long* out, double* ticks_per_second);
-/* The timeline as it stands while a launch may still run: copied on a stream of its own, so it
- * does not wait for the solver's stream (an end tick of 0 marks a system not finished yet;
- * enabling the timeline clears it). For progress reports of long launches. */
+/* The timeline as it stands while a launch may still run (an end tick of 0 marks a system not
+ * finished yet). The timeline lives in pinned host memory mapped into the device, so this
+ * reads it without issuing any GPU command. For progress reports of long launches. */
 int bex_peek_timeline(bex_solver* s, long long* out);
 
 /* BiCG warm start (no reference counterpart). params: BEX_MLP_NPARAM (4609) floats in the

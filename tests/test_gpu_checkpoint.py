@@ -108,3 +108,27 @@ def test_dense_index_bounds_are_checked_on_the_host():
     # System 1 started full: it stores nothing and keeps its index; system 0 fills its buffer.
     assert list(s.host_view(V.DenseIndex)) == [ndo, ndo]
     s.close()
+
+
+def test_timeline_and_progress_peek():
+    """The per-system timeline sits in host-mapped memory: finished systems can be counted
+    without a GPU command; enabling clears it, disabling frees it."""
+    ens = wl.config0().subset([9, 18, 27, 36])
+    print(f"case: timeline, {ens.NS} systems, tau in [0, 0.1]", flush=True)
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.host_view(V.TimeDomain)[ens.NS:] = 0.1
+    s.SynchroniseFromHostToDevice(V.All)
+    s.enable_timeline(True)
+    assert s.finished_systems() == 0
+    s.Solve()
+    s.SynchroniseDevice()
+    assert s.finished_systems() == ens.NS
+    t = s.timeline()
+    assert np.all(t[:, 1] > t[:, 0]) and np.all(t[:, 0] >= 0.0)
+    s.enable_timeline(True)
+    assert s.finished_systems() == 0
+    s.enable_timeline(False)
+    with pytest.raises(BexError):
+        s.finished_systems()
+    s.close()
