@@ -301,23 +301,36 @@ def tail_split(stats, k=64):
             "bulk": iters_per_rhs(stats, order[k:])}
 
 
-def wait_with_progress(pool, solvers, label, every=20.0):
-    """Waits for a pool launch; every `every` seconds it logs how many systems have finished,
-    read from the per-system timeline the kernels write (enabled before the launch)."""
+def wait_with_progress(pool, solvers, label, every=20.0, limit=None):
+    """Waits for a launch (pool or solver); every `every` seconds it logs how many systems have
+    finished, read from the per-system timeline the kernels write (enabled before the launch).
+    After `limit` seconds it asks the remaining systems to stop (bex_request_stop) and returns
+    the number that had finished by then (None if all finished within the limit)."""
     total = sum(s.NS for s in solvers)
-    last = time.perf_counter()
+    t0 = last = time.perf_counter()
+    finished_at_stop = None
     while True:
         done = sum(s.finished_systems() for s in solvers)
         if done >= total:
             break
-        if time.perf_counter() - last >= every:
+        now = time.perf_counter()
+        if limit is not None and finished_at_stop is None and now - t0 >= limit:
+            finished_at_stop = done
+            log(f"{label}: {done} of {total} systems finished after {limit:.0f} s; stopping "
+                "the others")
+            for s in solvers:
+                s.request_stop()
+        if now - last >= every:
             log(f"{label}: {done} of {total} systems finished")
-            last = time.perf_counter()
+            last = now
         time.sleep(0.05)
     pool.synchronise()
+    for s in solvers:
+        s.clear_stop()
+    return finished_at_stop
 
 
-def run_pool_leg(end_time, probe_fraction=0.02, sequential=True):
+def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None):
     """Config 2 (20480 systems, N in {8, 16, 32, 64, 128}, per-system coupling) through a
     ProblemPool: autotuned variants and queue order, one graph launch over [0, end_time] from
     rest, then (optionally) the same classes launched one after another."""
@@ -347,10 +360,16 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True):
     pool.synchronise()
     t0 = time.perf_counter()
     pool.Solve()
-    wait_with_progress(pool, solvers, "pool")
+    stopped_after = wait_with_progress(pool, solvers, "pool", limit=limit)
     wall = time.perf_counter() - t0
     stats = [sv.stats() for sv in solvers]
     slowest = {str(e.N): float(sv.timeline()[:, 1].max()) for sv, e in zip(solvers, classes)}
+    completed = {}
+    for sv, e in zip(solvers, classes):
+        sv.SynchroniseFromDeviceToHost(V.ActualTime)
+        sv.InsertSynchronisationPoint()
+        sv.SynchroniseSolver()
+        completed[str(e.N)] = int(np.count_nonzero(sv.host_view(V.ActualTime) >= end_time - 1e-9))
     flops = sum(flops_fm1(st, e.N) for st, e in zip(stats, classes))
     seq = {}
     for sv in solvers:
@@ -382,6 +401,10 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True):
                  f"{sum(sum(t.values()) for t in tuned.values()):.0f} ms of probes",
         "bicg_iterations_per_rhs": {str(e.N): iters_per_rhs(st) for st, e in zip(stats, classes)},
         "class_last_system_end_s": slowest,
+        "systems_completed": completed,
+        "stopped_after_s": None if stopped_after is None else limit,
+        "forced_accepts_per_system_max": {str(e.N): int(st[:, 7].max())
+                                          for st, e in zip(stats, classes)},
         "max_accepted_steps_per_system": {str(e.N): int(st[:, 0].max())
                                           for st, e in zip(stats, classes)},
         "forced_accepts": {str(e.N): int(st[:, 7].sum()) for st, e in zip(stats, classes)},
@@ -396,11 +419,12 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True):
     return out
 
 
-def run_second_period_leg():
+def run_second_period_leg(limit=None):
     """Headline workload's second period (continuing from the end of the first, no dense
     output): per-system finish times from the kernel timeline, so the launch's tail (the systems
     near the Keller-Miksis Mach-1 singularity) and its bulk are reported apart."""
     from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
     ens = wl.config1()
     s = make_headline_solver(ens, 1.0, dense=False)
@@ -412,23 +436,23 @@ def run_second_period_leg():
     s.SynchroniseDevice()
     t0 = time.perf_counter()
     s.Solve()
-    total, last = ens.NS, time.perf_counter()
-    while True:
-        done = s.finished_systems()
-        if done >= total:
-            break
-        if time.perf_counter() - last >= 20.0:
-            log(f"second period: {done} of {total} systems finished")
-            last = time.perf_counter()
-        time.sleep(0.05)
-    s.SynchroniseDevice()
+
+    class _Launch:  # wait_with_progress synchronises through this
+        def synchronise(self):
+            s.SynchroniseDevice()
+    stopped_after = wait_with_progress(_Launch(), [s], "second period", limit=limit)
     wall = time.perf_counter() - t0
     st = s.stats()
     end = s.timeline()[:, 1]
+    s.SynchroniseFromDeviceToHost(V.ActualTime)
+    s.InsertSynchronisationPoint()
+    s.SynchroniseSolver()
+    completed = int(np.count_nonzero(s.host_view(V.ActualTime) >= 1.0 - 1e-9))
     s.close()
     order = np.argsort(-end)
     return {"workload": "16384 systems x N=32, second period (continuing from the first)",
-            "wall_s": wall, "finish_s_quantiles": {q: float(np.quantile(end, float(q)))
+            "wall_s": wall, "systems_completed": completed,
+            "stopped_after_s": None if stopped_after is None else limit, "finish_s_quantiles": {q: float(np.quantile(end, float(q)))
                                                    for q in ("0.5", "0.9", "0.99", "0.999")},
             "last_systems": [{"system": int(k), "finish_s": float(end[k]),
                               "accepted": int(st[k, 0]), "rejected": int(st[k, 1]),
@@ -491,6 +515,9 @@ def main():
                     help="period fraction of the config-2 pool leg (0 skips it); a full period "
                          "is dominated by the few systems that reach the Mach-1 singularity "
                          "and takes minutes")
+    ap.add_argument("--leg-time-limit", type=float, default=None,
+                    help="with --leg: seconds after which the leg's remaining systems are asked "
+                         "to stop (their state is written back as it stands)")
     ap.add_argument("--config4-end-time", type=float, default=0.02,
                     help="period fraction of the N = 256 leg (0 skips it)")
     ap.add_argument("--end-time", type=float, default=1.0,
@@ -507,9 +534,10 @@ def main():
         return copy_calibration()
     if args.leg:
         log(f"{args.leg} leg alone")
-        rec = (run_pool_leg(args.pool_end_time, sequential=False) if args.leg == "pool"
+        rec = (run_pool_leg(args.pool_end_time, sequential=False, limit=args.leg_time_limit)
+               if args.leg == "pool"
                else run_config4_leg(args.config4_end_time) if args.leg == "n256"
-               else run_second_period_leg())
+               else run_second_period_leg(limit=args.leg_time_limit))
         print(json.dumps({"leg": args.leg, **rec}), flush=True)
         return 0
     if args.gpus != 1:

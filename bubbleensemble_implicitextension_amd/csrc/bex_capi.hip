@@ -63,6 +63,8 @@ struct bex_solver {
   // any GPU command, also while a launch runs); d_timeline is its device address.
   long long* h_timeline = nullptr;
   long long* d_timeline = nullptr;
+  int* h_stop = nullptr;  // stop flag in host-mapped memory (bex_request_stop)
+  int* d_stop = nullptr;  // its device address
 
   double InitialTimeStep = 1e-2, MaximumTimeStep = 1.0e6, MinimumTimeStep = 1.0e-12;
   double TimeStepGrowLimit = 5.0, TimeStepShrinkLimit = 0.1, DenseOutputMinimumTimeStep = 0.0;
@@ -124,25 +126,30 @@ struct KernelChoice {
   int block;
 };
 
-// Dense output compiles into separate kernels, so launches without it (NDO = 0) carry none of
-// its registers.
+// Dense output compiles into separate kernels (none, step points, Hermite), so launches without
+// it (NDO = 0) carry none of its registers and each mode only its own.
+template <class L, int ALG>
+KernelFn kernel_of(int dense) {
+  switch (dense) {
+    case bexk::kDenseSteps: return (KernelFn)bexk::solve_kernel<L, ALG, bexk::kDenseSteps>;
+    case bexk::kDenseHermite: return (KernelFn)bexk::solve_kernel<L, ALG, bexk::kDenseHermite>;
+    default: return (KernelFn)bexk::solve_kernel<L, ALG, bexk::kDenseNone>;
+  }
+}
 template <class L>
-KernelChoice choice(int alg, bool dense) {
-  KernelFn fn;
-  if (dense)
-    fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4, true>
-                        : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45, true>;
-  else
-    fn = alg == BEX_RK4 ? (KernelFn)bexk::solve_kernel<L, bexk::kRK4, false>
-                        : (KernelFn)bexk::solve_kernel<L, bexk::kRKCK45, false>;
+KernelChoice choice(int alg, int dense) {
+  const KernelFn fn = alg == BEX_RK4 ? kernel_of<L, bexk::kRK4>(dense) : kernel_of<L, bexk::kRKCK45>(dense);
   return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
 }
+
+// Dense-output kernel variant of a solver's current settings.
+int dense_mode(const bex_solver* s);
 
 // layout: BEX_LAYOUT_AUTO picks the role-pair layouts (32 units: one system per wave; 64 and
 // 128 units: two and eight waves per system, the matrix in registers) and the lane-per-unit
 // layouts otherwise; BEX_LAYOUT_GROUP forces lane-per-unit (the matrix in LDS). mode:
 // bexk::WarmStartMode; the learned guess and sample collection exist for 32 units only.
-bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice* k) {
+bool kernel_for(int ups, int alg, int layout, int mode, int dense, KernelChoice* k) {
   if ((ups == 64 || ups == 128) && layout != BEX_LAYOUT_GROUP) {
     if (mode != bexk::kPlain) return false;
     if (ups == 64)
@@ -165,7 +172,11 @@ bool kernel_for(int ups, int alg, int layout, int mode, bool dense, KernelChoice
     case 16: *k = choice<bexk::GroupLayout<16>>(alg, dense); return true;
     case 32: *k = choice<bexk::GroupLayout<32>>(alg, dense); return true;
     case 64: *k = choice<bexk::GroupLayout<64>>(alg, dense); return true;
+    case 96: *k = choice<bexk::WideLayout<96, false, 2>>(alg, dense); return true;
     case 128: *k = choice<bexk::WideLayout<128, false, 2>>(alg, dense); return true;
+    case 160: *k = choice<bexk::WideLayout<160, false, 2>>(alg, dense); return true;
+    case 192: *k = choice<bexk::WideLayout<192, false, 2>>(alg, dense); return true;
+    case 224: *k = choice<bexk::WideLayout<224, false, 2>>(alg, dense); return true;
     case 256: *k = choice<bexk::WideLayout<256, false, 2>>(alg, dense); return true;
     default: return false;
   }
@@ -188,6 +199,11 @@ __global__ __launch_bounds__(64) void mlp_mfma_kernel(const float* __restrict__ 
   for (int k = 0; k < BEX_MLP_IN; k++) wf[k] = (r < n) ? (double)rows[r * stride + k] : 0.0;
   const float y = bexk::mlp_factor(&snet, wf, lane);
   if (lane < 32 && r < n) out[r] = y;
+}
+
+int dense_mode(const bex_solver* s) {
+  if (s->d.NDO <= 0) return bexk::kDenseNone;
+  return (s->DenseInterpolation && s->d_DenseAux) ? bexk::kDenseHermite : bexk::kDenseSteps;
 }
 
 // Kernel arguments of one launch over the solver's buffers (also used by pools).
@@ -221,6 +237,7 @@ void fill_args(const bex_solver* s, BexKernelArgs* ap) {
   a.EventDirection = s->d_dir;
   a.Stats = s->d_stats;
   a.Timeline = s->d_timeline;
+  a.StopFlag = s->d_stop;
   a.InitialTimeStep = s->InitialTimeStep;
   a.MaximumTimeStep = s->MaximumTimeStep;
   a.MinimumTimeStep = s->MinimumTimeStep;
@@ -456,9 +473,10 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
       (d.CCI != 0 && d.CCI != 1) || (d.Algorithm != BEX_RK4 && d.Algorithm != BEX_RKCK45))
     return fail(nullptr, BEX_E_ARGUMENT, "bex_create: invalid problem description");
   KernelChoice kc;
-  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, false, &kc))
+  if (!kernel_for(d.UPS, d.Algorithm, BEX_LAYOUT_AUTO, bexk::kPlain, bexk::kDenseNone, &kc))
     return fail(nullptr, BEX_E_UNSUPPORTED,
-                "bex_create: units per system must be 8, 16, 32, 64, 128 or 256 in this build");
+                "bex_create: units per system must be 8, 16 or a multiple of 32 up to 256 in "
+                "this build");
 
   bex_solver* s = new bex_solver();
   s->d = d;
@@ -540,6 +558,10 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   ok = ok && hipMalloc((void**)&s->d_tol, sizeof(double) * ntol) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_dir, sizeof(int) * (d.NE > 0 ? d.NE : 1)) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_stats, sizeof(long long) * (size_t)d.NS * BEX_NSTAT) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&s->h_stop, sizeof(int),
+                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer((void**)&s->d_stop, s->h_stop, 0) == hipSuccess;
+  if (ok) *s->h_stop = 0;
   if (!ok) {
     g_create_error = "bex_create: memory allocation failed";
     bex_destroy(s);
@@ -595,6 +617,7 @@ int bex_destroy(bex_solver* s) {
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
   if (s->h_timeline) hipHostFree(s->h_timeline);
+  if (s->h_stop) hipHostFree(s->h_stop);
   if (s->d_DenseAux) hipFree(s->d_DenseAux);
   if (s->d_net) hipFree(s->d_net);
   if (s->d_samples) hipFree(s->d_samples);
@@ -636,14 +659,14 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_PriorityStepThreshold: s->PriorityStepThreshold = (int)v; break;
     case BEX_KernelLayout: {
       KernelChoice kc;
-      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, false, &kc))
+      if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, bexk::kDenseNone, &kc))
         return fail(s, BEX_E_UNSUPPORTED, "SolverOption: no kernel layout for this unit count");
       s->KernelLayout = (int)v;
       break;
     }
     case BEX_WarmStart: {
       KernelChoice kc;
-      if (v != 0.0 && !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kLearned, false, &kc))
+      if (v != 0.0 && !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kLearned, bexk::kDenseNone, &kc))
         return fail(s, BEX_E_UNSUPPORTED,
                     "SolverOption: the learned warm start needs 32 units and the role-pair layout");
       s->WarmStart = (v != 0.0) ? 1 : 0;
@@ -976,7 +999,7 @@ int bex_solve(bex_solver* s) {
   const int mode = s->WarmStart ? bexk::kLearned : (s->d_samples ? bexk::kSampling : bexk::kPlain);
   if (mode == bexk::kLearned && !s->d_net)
     return fail(s, BEX_E_ARGUMENT, "Solve: warm start enabled but no network was set");
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, s->d.NDO > 0, &kc))
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, dense_mode(s), &kc))
     return fail(s, BEX_E_UNSUPPORTED, "Solve: no kernel for this unit count, layout and mode");
   BexKernelArgs a{};
   fill_args(s, &a);
@@ -1123,6 +1146,18 @@ int bex_peek_timeline(bex_solver* s, long long* out) {
   return BEX_OK;
 }
 
+int bex_request_stop(bex_solver* s) {
+  if (!s || !s->h_stop) return BEX_E_ARGUMENT;
+  *reinterpret_cast<volatile int*>(s->h_stop) = 1;
+  return BEX_OK;
+}
+
+int bex_clear_stop(bex_solver* s) {
+  if (!s || !s->h_stop) return BEX_E_ARGUMENT;
+  *reinterpret_cast<volatile int*>(s->h_stop) = 0;
+  return BEX_OK;
+}
+
 int bex_reset_stats(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
@@ -1177,7 +1212,7 @@ int bex_enable_sampling(bex_solver* s, long long capacity, int every) {
   s->SampleEvery = 0;
   if (capacity == 0) return BEX_OK;
   KernelChoice kc;
-  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kSampling, false, &kc))
+  if (!kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, bexk::kSampling, bexk::kDenseNone, &kc))
     return fail(s, BEX_E_UNSUPPORTED,
                 "bex_enable_sampling: sample collection needs 32 units and the role-pair layout");
   BEX_HIP(s, hipMalloc((void**)&s->d_samples, sizeof(float) * BEX_SAMPLE_FLOATS * (size_t)capacity));
@@ -1356,10 +1391,21 @@ int capture(bex_pool* p) {
     hipGraphDestroy(p->graph);
     p->graph = nullptr;
   }
+  // Branches in descending estimated cost (N^2 per unit of BiCG work, times the class size):
+  // when the runtime maps more branches than it has hardware queues onto one queue, the
+  // costliest class is then never the one left waiting behind another.
+  std::vector<size_t> order(p->cls.size());
+  for (size_t c = 0; c < order.size(); c++) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    const double cx = (double)p->cls[x].N * p->cls[x].N * p->cls[x].NS;
+    const double cy = (double)p->cls[y].N * p->cls[y].N * p->cls[y].NS;
+    return cx > cy;
+  });
   BEX_PHIP(p, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
   int rc = BEX_OK;
   if (hipEventRecord(p->fork, p->stream) != hipSuccess) rc = BEX_E_HIP;
-  for (size_t c = 0; rc == BEX_OK && c < p->cls.size(); c++) {
+  for (size_t c : order) {
+    if (rc != BEX_OK) break;
     if (hipStreamWaitEvent(p->side[c], p->fork, 0) != hipSuccess) {
       rc = BEX_E_HIP;
       break;

@@ -556,26 +556,44 @@ __device__ __forceinline__ double row_dot4_pair(MF m, ZF z, int h) {
   return h == 0 ? pair + other : other + pair;
 }
 
-// N = 128 and 256: one system per workgroup, LPU lanes per unit (N * LPU / 64 waves).
-// Reductions combine the waves through LDS (the tree's cross-wave levels, s[i] op s[i + 64] for
-// N = 128 and (s[i] op s[i + 128]) op (s[i + 64] op s[i + 192]) for N = 256) and finish with
-// an in-wave tree that every wave evaluates identically; system-scope synchronisation is the
-// workgroup barrier. N = 128 keeps the system's matrix in LDS (132 KB, one system per CU); a
-// 256 x 256 FP64 matrix (512 KB) exceeds the CU's LDS, so N = 256 streams M and M^T
+// Levels of the reference's reduction fold (Steppers.cuh:745-805) above 64 entries, for N a
+// multiple of 32 from 96 to 256: the 32-unit warps beyond the largest power of two, nP2, fold
+// onto the first ones (s[i] op s[i + 32 nP2] for i < 32 (N/32 - nP2)), then the halving levels
+// run down to 64 entries. Returns entry l < 64 of that partial tree from the N values in red.
+template <int N, class Op>
+__device__ __forceinline__ double fold_to_64(const double* red, int l, Op op) {
+  constexpr int nFull = N / 32;
+  constexpr int nP2 = nFull >= 8 ? 8 : (nFull >= 4 ? 4 : 2);
+  constexpr int lim = 32 * nP2;
+  constexpr int folded = 32 * (nFull - nP2);
+  auto t = [&](int i) { return i < folded ? op(red[i], red[i + lim]) : red[i]; };
+  if (lim == 64) return t(l);
+  if (lim == 128) return op(t(l), t(l + 64));
+  return op(op(t(l), t(l + 128)), op(t(l + 64), t(l + 192)));
+}
+
+// N a multiple of 32 from 96 to 256: one system per workgroup, LPU lanes per unit
+// (N * LPU / 64 waves, whole waves only). Reductions combine the waves through LDS (the fold's
+// levels above 64 entries, fold_to_64) and finish with an in-wave tree that every wave
+// evaluates identically; system-scope synchronisation is the workgroup barrier. N <= 128 keeps
+// the system's matrix in LDS (132 KB for N = 128, one system per CU); larger matrices
+// (a 256 x 256 FP64 matrix is 512 KB) exceed the CU's LDS, so they stream M and M^T
 // (column-major, both products coalesced) from global memory through L2 and MALL. With two
 // lanes per unit, each lane runs half of a row's FMA chains (row_dot4_pair) and all four SIMDs
 // of the CU take part; the per-unit work is duplicated on the unit's lanes and the first lane
 // ("primal") writes results.
 template <int N, bool PS = true, int LPU = 1>
 struct WideLayout {
-  static_assert(N == 128 || N == 256, "multi-wave layout: 128 or 256 units");
+  static_assert(N % 32 == 0 && N >= 96 && N <= 256, "multi-wave layout: 96 to 256 units");
   static_assert(LPU == 1 || LPU == 2, "one or two lanes per unit");
-  static constexpr bool kLdsM = (N == 128);
+  static_assert((N * LPU) % 64 == 0, "whole waves only");
+  static constexpr bool kLdsM = (N <= 128);
   static constexpr int kUnits = N;
   static constexpr int kBlock = N * LPU;
   static constexpr int kWaves = kBlock / kWave;
   static constexpr int kSystemsPerBlock = 1;
-  static constexpr int kWavesPerEU = kLdsM ? 1 : 2;  // N = 128: LDS holds one system per CU
+  // N = 128: LDS holds one system per CU; N = 96 (74 KB of matrix) two.
+  static constexpr int kWavesPerEU = (N == 128) ? 1 : 2;
   static constexpr int kChunk = 16;
   static constexpr int kMode = kPlain;
   static constexpr bool kPerSystemM = PS;
@@ -645,9 +663,7 @@ struct WideLayout {
     if (h == 0) red[u] = v;
     __syncthreads();
     const int l = tid & (kWave - 1);
-    const double first = (N == 128) ? op(red[l], red[l + 64])
-                                    : op(op(red[l], red[l + 128]), op(red[l + 64], red[l + 192]));
-    const double w = group_tree<kWave>(first, op, 0);
+    const double w = group_tree<kWave>(fold_to_64<N>(red, l, op), op, 0);
     __syncthreads();
     return w;
   }
@@ -848,8 +864,17 @@ struct RolePairLayout32 {
   static constexpr int kItemsPerBlock = 1;
   static constexpr int kSystemsPerItem = 1;
 
+  // The plain kernel keeps the unit parameters in LDS ([unit][NUP], one copy for both halves,
+  // 5.4 KB per wave with the Keller-Miksis model): the right-hand side reads each once per
+  // stage, and the ~42 registers they would hold across the time loop are what the stage
+  // state otherwise spills for at 3 waves per SIMD.
+  static constexpr bool kUnitParamsInLds = (MODE == kPlain);
+  static constexpr int kNup = BexSystemDims::NUP > 0 ? BexSystemDims::NUP : 1;
+
   static constexpr size_t lds_bytes() {
-    return (MODE == kLearned ? sizeof(BexMlpDevice) : 0) + sizeof(Sys) + sizeof(int) * BEX_NSTAT;
+    return (MODE == kLearned ? sizeof(BexMlpDevice) : 0) +
+           (kUnitParamsInLds ? sizeof(double) * 32 * kNup : 0) + sizeof(Sys) +
+           sizeof(int) * BEX_NSTAT;
   }
 
   int lane, sid_;
@@ -857,7 +882,10 @@ struct RolePairLayout32 {
   double mrow[32];  // row i of M (primal) or of M^T (shadow)
   Sys* sys;
   int* stat;
+  double* upar_lds;  // [32][NUP] when kUnitParamsInLds
   const BexMlpDevice* net;
+
+  __device__ __forceinline__ double* unit_params() const { return upar_lds + (lane & 31) * kNup; }
 
   __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs& a) {
     lane = threadIdx.x;
@@ -871,6 +899,8 @@ struct RolePairLayout32 {
       net = n;
       p += sizeof(BexMlpDevice);
     }
+    upar_lds = reinterpret_cast<double*>(p);
+    if (kUnitParamsInLds) p += sizeof(double) * 32 * kNup;
     sys = reinterpret_cast<Sys*>(p);
     stat = reinterpret_cast<int*>(sys + 1);
   }
@@ -1526,8 +1556,24 @@ __device__ __forceinline__ void count(int* st, int k, int v) { atomicAdd(&st[k],
 // =========================================================================================
 // Time loop of one system (Solver.cuh:11-651), shared by the layouts.
 
-// DENSE = false compiles a kernel without any dense-output code (launches with NDO = 0).
-template <class L, int ALG, bool DENSE>
+template <class L, class = void>
+struct UnitParamsInLds {
+  static constexpr bool value = false;
+};
+template <class L>
+struct UnitParamsInLds<L, decltype(void(L::kUnitParamsInLds))> {
+  static constexpr bool value = L::kUnitParamsInLds;
+};
+template <class L>
+__device__ __forceinline__ constexpr bool unit_params_in_lds() {
+  return UnitParamsInLds<L>::value;
+}
+
+// Dense-output variants, compiled as separate kernels so that each carries only its own mode's
+// registers: none (NDO = 0), the reference's step-point sampling, Hermite interpolation.
+enum DenseMode { kDenseNone = 0, kDenseSteps = 1, kDenseHermite = 2 };
+
+template <class L, int ALG, int DENSE>
 __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   using D = BexSystemDims;
   constexpr int N = L::kUnits;
@@ -1565,13 +1611,15 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     S.UDT = 0;
     // The host keeps the index in [0, NDO]; anything outside is treated as a full buffer (no
     // store), never used as an offset.
-    const int di = (DENSE && a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
+    const int di = (DENSE != kDenseNone && a.NDO > 0) ? a.DenseOutputIndex[sid] : 0;
     S.DOIDX = (di < 0 || di > a.NDO) ? a.NDO : di;
   }
 
-  // Unit scope into registers / private memory (Solver.cuh:234-257).
+  // Unit scope into registers / private memory (Solver.cuh:234-257); layouts that set
+  // kUnitParamsInLds keep the unit parameters in LDS instead.
   double x[UD], xn[UD], err[UD];
-  double upar[NUP > 0 ? NUP : 1];
+  double upar_regs[unit_params_in_lds<L>() ? 1 : (NUP > 0 ? NUP : 1)];
+  double* upar = upar_regs;
   double uacc[NUA > 0 ? NUA : 1];
   int iuacc[NiUA > 0 ? NiUA : 1];
   double ea[NE > 0 ? NE : 1], en[NE > 0 ? NE : 1];
@@ -1583,8 +1631,16 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     rtol[i] = a.RelativeTolerance[i];
     atol[i] = a.AbsoluteTolerance[i];
   }
+  if constexpr (unit_params_in_lds<L>()) {
+    upar = lay.unit_params();
+    if (primal) {
 #pragma unroll
-  for (int i = 0; i < NUP; i++) upar[i] = a.UnitParameters[gtid + i * TLT];
+      for (int i = 0; i < NUP; i++) upar[i] = a.UnitParameters[gtid + i * TLT];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NUP; i++) upar[i] = a.UnitParameters[gtid + i * TLT];
+  }
 #pragma nounroll
   for (int i = 0; i < NUA; i++) uacc[i] = a.UnitAccessories[gtid + i * TLT];
 #pragma unroll
@@ -1633,7 +1689,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
     if ((updateDO == 0) && (updateStep == 1)) skipped++;
   };
-  if (DENSE && a.NDO > 0) {
+  if (DENSE != kDenseNone && a.NDO > 0) {
     updateDO = (S.DOIDX < a.NDO) ? 1 : 0;
     store_dense(0);
   }
@@ -1646,7 +1702,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   // derivative wait in DenseAux ([2 UD][TLT]).
   // The interval and grid bookkeeping lives in the system's LDS record (uniform values; kept
   // out of the registers that hold the stage vectors).
-  const bool interp = DENSE && (a.NDO > 0) && (a.DenseInterpolation != 0);
+  const bool interp = (DENSE == kDenseHermite) && (a.NDO > 0);
   if (uid == 0) {
     S.pending = 0;
     S.flushOnly = 0;
@@ -1696,7 +1752,18 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   // takes its next system from a queue starts it at the base level again.
   __builtin_amdgcn_s_setprio(0);
   long long stage_count = 0;  // RK stages of this launch (sample selection)
+  int polls = 0;
   while (terminate == 0) {  // Solver.cuh:329-572
+    // Cooperative stop (no reference counterpart): every 256 attempts the system reads the
+    // host's stop flag; a set flag ends the system here, as a user termination would, with its
+    // state and time written back as they stand. The decision is taken for the whole system.
+    if (a.StopFlag != nullptr && (++polls & 255) == 0) {
+      const int req = *reinterpret_cast<const volatile int*>(a.StopFlag);
+      if (lay.any(req != 0)) {
+        terminate = 1;
+        continue;
+      }
+    }
     int updateStep = 1, endReached = 0;
     dt = dtNew;
     if (dt > (TD1 - t)) {
@@ -1979,7 +2046,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
     const int udt = S.UDT;
 
-    if (DENSE && a.NDO > 0 && !interp) {  // Solver.cuh:528-557
+    if (DENSE == kDenseSteps && a.NDO > 0) {  // Solver.cuh:528-557
       if (updateStep == 1) {
         const int idx = S.DOIDX;
         updateDO = ((idx < a.NDO) && (doTime < t) && (skipped >= (a.DenseOutputSaveFrequency - 1)))
@@ -2028,7 +2095,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     a.TimeDomain[sid + NS] = S.TD[1];
     for (int i = 0; i < D::NSA; i++) a.SystemAccessories[sid + i * NS] = S.sACC[i];
     for (int i = 0; i < D::NiSA; i++) a.IntegerSystemAccessories[sid + i * NS] = S.isACC[i];
-    if (DENSE && a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
+    if (DENSE != kDenseNone && a.NDO > 0) a.DenseOutputIndex[sid] = S.DOIDX;
   }
   lay.sync();
   if (primal)
@@ -2036,7 +2103,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   if (leader && a.Timeline) a.Timeline[2 * (long long)sid + 1] = (long long)wall_clock64();
 }
 
-template <class L, int ALG, bool DENSE>
+template <class L, int ALG, int DENSE>
 __global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
 void solve_kernel(const BexKernelArgs a, long long count) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2068,7 +2135,7 @@ void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long
     lay.sync();
     if (item >= nitems) break;
     lay.bind(a, order, item, count);
-    integrate<L, ALG, false>(lay, a);  // pools run without dense output
+    integrate<L, ALG, kDenseNone>(lay, a);  // pools run without dense output
     lay.sync();
   }
 }

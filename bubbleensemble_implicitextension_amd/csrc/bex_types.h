@@ -56,6 +56,9 @@ struct BexKernelArgs {
   const int* EventDirection;
   long long* Stats;  // [NS][BEX_NSTAT], accumulated
   long long* Timeline;  // optional [NS][2]: device wall-clock ticks at system start and end
+  // Host-mapped stop flag: while it is nonzero, every running system ends at its next check
+  // (every 256 step attempts) with its state and time written back as they stand.
+  const int* StopFlag;
 
   double InitialTimeStep;
   double MaximumTimeStep;

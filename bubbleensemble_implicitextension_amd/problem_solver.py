@@ -282,6 +282,14 @@ class ProblemSolver:
             t -= t[valid, 0].min()
         return t / tps.value
 
+    def request_stop(self):
+        """Asks every running system to end at its next check (every 256 step attempts), state
+        written back as it stands; stays in force until clear_stop()."""
+        self._check(self._lib.bex_request_stop(self._h))
+
+    def clear_stop(self):
+        self._check(self._lib.bex_clear_stop(self._h))
+
     def finished_systems(self):
         """Systems of the current (or last) launch that have finished, read while the launch
         may still run (needs enable_timeline, which clears the count)."""

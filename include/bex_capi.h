@@ -171,6 +171,15 @@ int bex_synchronise_solver(bex_solver* s);
 int bex_checkpoint_save(bex_solver* s);
 int bex_checkpoint_restore(bex_solver* s);
 
+/* Cooperative stop (no reference counterpart): bex_request_stop sets a flag in host-mapped
+ * memory that every running system of this solver reads every 256 step attempts; a system
+ * that sees it set ends there, as if the user had terminated it, with its state, time and
+ * accessories written back as they stand (ActualTime < TimeDomain[1] marks it). The flag stays
+ * set, stopping later launches too, until bex_clear_stop. For time-bounded runs of workloads
+ * whose stiffest systems need millions of steps. */
+int bex_request_stop(bex_solver* s);
+int bex_clear_stop(bex_solver* s);
+
 /* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);
  * copies device counters to `out` (synchronous). No reference counterpart. */
 int bex_get_stats(bex_solver* s, long long* out);

@@ -747,7 +747,9 @@ extern "C" int bex_oracle_solve(const bex_oracle_dims* d, const bex_oracle_optio
   if (!d || !o || !b || !stats) return -1;
   if (d->UD != 2 || d->N <= 0 || d->NS <= 0) return -2;
   if (d->N < 32 && (d->N & (d->N - 1)) != 0) return -3;
-  if (mode == BEX_ORACLE_CANONICAL && (d->N & (d->N - 1)) != 0) return -3;
+  // Canonical reductions: halving trees below 32 units, the reference's fold for multiples of
+  // 32 (what the kernels implement).
+  if (mode == BEX_ORACLE_CANONICAL && d->N >= 32 && d->N % 32 != 0) return -3;
   if (d->NUP < 13 || d->NUA < 2 || d->NiUA < 1) return -4;  // Keller-Miksis hooks
   Ctx c{d, o, b, stats};
   if (mode == BEX_ORACLE_LITERAL)

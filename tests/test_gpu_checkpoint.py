@@ -132,3 +132,37 @@ def test_timeline_and_progress_peek():
     with pytest.raises(BexError):
         s.finished_systems()
     s.close()
+
+
+def test_stop_request_ends_systems_at_their_next_check():
+    """A stop requested before the launch ends every system at its first check, after exactly
+    255 step attempts, with ActualTime short of the time domain's end; systems that finish in
+    fewer attempts are not affected. After clear_stop the same problem runs to the end."""
+    ens = wl.config0().subset([9, 18, 27, 36, 0])
+    print(f"case: stop request, {ens.NS} systems, tau in [0, 0.3]", flush=True)
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.host_view(V.TimeDomain)[ens.NS:] = 0.3
+    s.SynchroniseFromHostToDevice(V.All)
+    s.checkpoint_save()
+    s.request_stop()
+    s.Solve()
+    s.SynchroniseFromDeviceToHost(V.ActualTime)
+    s.InsertSynchronisationPoint()
+    s.SynchroniseSolver()
+    st = s.stats()
+    t_end = s.host_view(V.ActualTime).copy()
+    stopped = t_end < 0.3 - 1e-9
+    assert stopped.any()
+    assert np.all(st[stopped, 10] == 255), st[:, 10]
+    assert np.all(st[~stopped, 10] <= 255)
+    s.clear_stop()
+    s.reset_stats()
+    s.checkpoint_restore()
+    s.Solve()
+    s.SynchroniseFromDeviceToHost(V.ActualTime)
+    s.InsertSynchronisationPoint()
+    s.SynchroniseSolver()
+    assert np.all(np.abs(s.host_view(V.ActualTime) - 0.3) < 1e-12)
+    assert np.all(s.stats()[stopped, 10] > 255)
+    s.close()

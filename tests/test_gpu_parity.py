@@ -31,10 +31,14 @@ def test_small_ensembles_bitexact(oracle, N, layout):
 
 
 @pytest.mark.parametrize("N,end,layout", [(128, 0.2, LAYOUT_ROLE_PAIR), (128, 0.2, LAYOUT_GROUP),
-                                           (256, 0.05, LAYOUT_GROUP)])
+                                           (256, 0.05, LAYOUT_GROUP), (96, 0.2, LAYOUT_GROUP),
+                                           (160, 0.05, LAYOUT_GROUP), (224, 0.03, LAYOUT_GROUP)])
 def test_wide_layouts_bitexact(oracle, N, end, layout):
     """Shared coupling matrix, multi-wave systems: N = 128 with the matrix in registers (eight
-    role-pair waves, two FMA chains each) or in LDS (lane per unit), N = 256 (M streamed)."""
+    role-pair waves, two FMA chains each) or in LDS (lane per unit), N = 256 (M streamed), and
+    unit counts that are multiples of 32 but not powers of two (N = 96 with M in LDS, 160 and
+    224 streamed), whose reductions follow the reference's fold of the extra 32-unit warps
+    (Steppers.cuh:745-805)."""
     ens = wl.small_ensemble(n=N, ns=3, seed=3 + N)
     print(f"small ensemble N={N} NS={ens.NS} end={end} layout={layout}", flush=True)
     s = run_gpu(ens, periods=1, end_time=end, layout=layout)

@@ -221,7 +221,7 @@ def _coupled_problem(N, seed, strength=1.0):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("N", [8, 32, 64])
+@pytest.mark.parametrize("N", [8, 32, 64, 96])
 def test_bicg_converges_to_lapack(oracle, mode, N):
     M, x1, rD, cpt, xu, A, b = _coupled_problem(N, seed=N)
     x, it = oracle.bicg(M, x1, rD, cpt, xu, rho_tol=0.0, res_tol=1e-12, max_iter=200, mode=mode)
