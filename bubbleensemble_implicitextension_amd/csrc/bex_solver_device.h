@@ -1569,6 +1569,18 @@ __device__ __forceinline__ constexpr bool unit_params_in_lds() {
   return UnitParamsInLds<L>::value;
 }
 
+// Whether the system definition declares that its two action hooks write only the unit's
+// accessories and no other hook reads them (BexSystemDims::kActionsWriteOnlyAccessories; absent
+// means no).
+template <class D, class = void>
+struct ActionsWriteOnlyAccessories {
+  static constexpr bool value = false;
+};
+template <class D>
+struct ActionsWriteOnlyAccessories<D, decltype(void(D::kActionsWriteOnlyAccessories))> {
+  static constexpr bool value = D::kActionsWriteOnlyAccessories;
+};
+
 // Dense-output variants, compiled as separate kernels so that each carries only its own mode's
 // registers: none (NDO = 0), the reference's step-point sampling, Hermite interpolation.
 enum DenseMode { kDenseNone = 0, kDenseSteps = 1, kDenseHermite = 2 };
@@ -1593,6 +1605,11 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   const int uid = lay.uid();
   const bool primal = lay.primal();
   const bool leader = primal && uid == 0;
+  // Lanes that duplicate another lane's unit (the shadow half of a role pair, the second lane of
+  // a lane pair) skip the action hooks when those only write accessories nobody else reads:
+  // their accessory copies are never written back, and each skipped hook saves a private-memory
+  // access of the unit's accessory array.
+  const bool run_actions = primal || !ActionsWriteOnlyAccessories<D>::value;
   const long long TLT = a.TLT;
   const long long gtid = (long long)(sid / a.ref_spb) * a.ref_block_stride +
                          (long long)(sid % a.ref_spb) * N + uid;  // Interface.cuh:875-879
@@ -1983,8 +2000,14 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
     }
 
     if (updateStep == 1) {  // Solver.cuh:433-525
+      // Interpolated dense output: the step's start point and derivative are kept only when the
+      // next output instant falls inside the step (a few dozen of the thousands of steps of a
+      // period), since only then is the step flushed.
+      bool covers = false;
       if (interp) {
-        if (primal) {
+        covers = (S.DOIDX < a.NDO) &&
+                 (S.gridT0 + (double)S.kq * a.DenseOutputMinimumTimeStep <= t + dt);
+        if (covers && primal) {
 #pragma unroll
           for (int i = 0; i < UD; i++) {
             a.DenseAux[gtid + i * TLT] = x[i];
@@ -2009,18 +2032,20 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
         S.dT = dt;
       }
       lay.sync();
-      CoupledSystems_PerBlock_ActionAfterSuccessfulTimeStep<double>(
-          sid, uid, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
-          a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
+      if (run_actions)
+        CoupledSystems_PerBlock_ActionAfterSuccessfulTimeStep<double>(
+            sid, uid, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
+            a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
       lay.sync();
       if (NE > 0) {
 #pragma unroll
         for (int i = 0; i < NE; i++) {
           if (((ea[i] > etol[i]) && (bex::fabs_d(en[i]) < etol[i]) && (edir[i] <= 0)) ||
               ((ea[i] < -etol[i]) && (bex::fabs_d(en[i]) < etol[i]) && (edir[i] >= 0))) {
-            CoupledSystems_PerBlock_ActionAfterEventDetection<double>(
-                sid, uid, i, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
-                a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
+            if (run_actions)
+              CoupledSystems_PerBlock_ActionAfterEventDetection<double>(
+                  sid, uid, i, S.UDT, S.T, S.dT, S.TD, x, upar, S.sPAR, a.GlobalParameters,
+                  a.IntegerGlobalParameters, uacc, iuacc, S.sACC, S.isACC);
             if (primal) atomicAdd(&st[BEX_STAT_EVENTS], 1);
           }
         }
@@ -2037,7 +2062,7 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
       if (interp) {
         if (uid == 0) {
           S.tB = t;
-          S.pending = 1;
+          S.pending = covers ? 1 : 0;
         }
         lay.sync();
       }

@@ -34,6 +34,10 @@ struct BexSystemDims {
   static constexpr int NiSA = 0;
   static constexpr int NE = 1;            // dR/dtau = 0
   static constexpr int NC = 1;
+  // ActionAfterSuccessfulTimeStep and ActionAfterEventDetection write nothing but the unit's
+  // accessories, and no other hook reads the accessories: the solver may then run the two
+  // action hooks only on the one lane per unit whose accessories it writes back.
+  static constexpr bool kActionsWriteOnlyAccessories = true;
 };
 
 // SYSTEM (BubbleEnsemble_SystemDefinition.cuh:7-49)

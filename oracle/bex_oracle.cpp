@@ -657,11 +657,17 @@ void solve_system(const Ctx& c, int sid) {
 
     // Update (Solver.cuh:433-441)
     if (updateStep == 1) {
+      // The step's start point and derivative are kept only when the next output instant falls
+      // inside the step; only such a step is flushed.
+      bool covers = false;
       if (interp) {
-        for (int i = 0; i < n2; i++) {
-          YA[i] = X[i];
-          FA[i] = K[i];  // k1 of the accepted step
-        }
+        covers = (doIdx < d.NDO) &&
+                 (gridT0 + (double)kq * o.DenseOutputMinimumTimeStep <= t + dt);
+        if (covers)
+          for (int i = 0; i < n2; i++) {
+            YA[i] = X[i];
+            FA[i] = K[i];  // k1 of the accepted step
+          }
         tA = t;
       }
       t += dt;
@@ -688,7 +694,7 @@ void solve_system(const Ctx& c, int sid) {
       }
       if (interp) {
         tB = t;
-        pending = true;
+        pending = covers;
       }
     } else {
       st[BEX_STAT_REJECTED]++;
