@@ -106,6 +106,9 @@ _SIGNATURES = {
     "bex_mlp_predict_mfma": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
                               ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p],
                              ctypes.c_int),
+    "bex_mlp_predict_mfma_noalloc": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
+                                      ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                      ctypes.c_void_p], ctypes.c_int),
     "bex_pool_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                          ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "bex_pool_destroy": ([ctypes.c_void_p], ctypes.c_int),

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1162,6 +1163,60 @@ int bex_reset_stats(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
   BEX_HIP(s, hipMemsetAsync(s->d_stats, 0, sizeof(long long) * (size_t)s->d.NS * BEX_NSTAT, s->stream));
+  return BEX_OK;
+}
+
+}  // extern "C"
+
+// Scratch of bex_mlp_predict_mfma_noalloc: one network packed in the MFMA operand layout
+// (sizeof(BexMlpDevice) = 9744 bytes), the staging buffer that bex_mlp_predict_mfma allocates
+// and frees on every call. Each device holds its own copy until the process ends; every call on
+// a device overwrites it, so calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[sizeof(BexMlpDevice)];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};  // device address of `scratch`, per device
+
+// Device address of the current device's `scratch`, looked up once per device (with that
+// device current) by hipGetSymbolAddress; host code never uses the symbol itself.
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return BEX_E_HIP;
+  if (dev < 0 || dev >= kScratchDevices) return BEX_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BEX_E_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bex_mlp_predict_mfma_noalloc(const float* rows, int stride, long long n, const float* params,
+                                 float* out, void* stream) {
+  if (!rows || !params || !out || n <= 0 || stride < BEX_MLP_IN) return BEX_E_ARGUMENT;
+  const long long nb = (n + 31) / 32;
+  if (nb > 0x7fffffff) return BEX_E_UNSUPPORTED;  // more rows than one launch covers
+  void* d = nullptr;
+  int rc = scratch_address(&d);
+  if (rc != BEX_OK) return rc;
+  BexMlpDevice host;
+  bex_mlp_pack_device(params, &host);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(d, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice, st) != hipSuccess)
+    return BEX_E_HIP;
+  hipLaunchKernelGGL(mlp_mfma_kernel, dim3((unsigned)nb), dim3(64), 0, st, rows, stride, n,
+                     static_cast<const BexMlpDevice*>(d), out);
+  // The packed copy lives in this frame: finish before returning, like bex_mlp_predict_mfma.
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return BEX_E_HIP;
   return BEX_OK;
 }
 

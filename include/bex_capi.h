@@ -232,6 +232,15 @@ int bex_mlp_predict(const float* rows, int stride, long long n, const float* par
  * pointer (4609 floats), rows/out device pointers; synchronous. */
 int bex_mlp_predict_mfma(const float* rows, int stride, long long n, const float* params,
                          float* out, void* stream);
+/* The same inference without any allocation: no reference counterpart; for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * it). The packed network goes to a library-owned static device buffer of 9744 bytes per device
+ * (one network) instead of a per-call hipMalloc; rows and out are the caller's device buffers
+ * as above; rows beyond one launch's reach (2^31 - 1 blocks of 32) are rejected with
+ * BEX_E_UNSUPPORTED. Every call on a device overwrites that device's buffer, so calls that
+ * overlap on one device race. Synchronous. */
+int bex_mlp_predict_mfma_noalloc(const float* rows, int stride, long long n, const float* params,
+                                 float* out, void* stream);
 
 /* Pools of uneven ensembles (no reference counterpart). A pool holds `nclass` size classes;
  * class c is a solver (bex_pool_class_solver: options, Set/Get, host views, synchronisation) of
