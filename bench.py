@@ -110,30 +110,35 @@ def cpu_baseline(sample=2048, seed=7):
     from oracle import pyoracle
 
     pyoracle.build()
+    native = pyoracle.build_native()
+    build = ("g++ -O3 -march=native -fopenmp, built on this host" if native else
+             "g++ -O3 -march=x86-64-v3 -fopenmp (the -march=native build failed)")
     cpu = host_cpu()
     threads = max(1, min(cpu["usable_cpus"], 16))
     ens = wl.config1()
     rng = np.random.default_rng(seed)
     sids = np.sort(rng.choice(ens.NS, size=sample, replace=False))
     p = pyoracle.OracleProblem.from_ensemble(ens.subset(sids))
-    t0 = time.perf_counter()
-    p.solve(mode=pyoracle.LITERAL, nthreads=threads)
-    dt = time.perf_counter() - t0
     # One thread on every 32nd system of the same sample (64 systems over the whole grid; the
     # sample is sorted by system index, i.e. by frequency, so a prefix would be the slowest).
     one_sids = sids[::max(1, sample // 64)][:64]
     one = pyoracle.OracleProblem.from_ensemble(ens.subset(one_sids))
-    t1 = time.perf_counter()
-    one.solve(mode=pyoracle.LITERAL, nthreads=1)
-    dt1 = time.perf_counter() - t1
+    with pyoracle.using_library(native or pyoracle.LIB_PATH):
+        t0 = time.perf_counter()
+        p.solve(mode=pyoracle.LITERAL, nthreads=threads)
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        one.solve(mode=pyoracle.LITERAL, nthreads=1)
+        dt1 = time.perf_counter() - t1
     return {"value": sample / dt, "unit": "system-periods/s", "cores": threads,
             "kind": "port",
             "sample": f"{sample} seeded-random systems (seed {seed}) of the 16384-system "
                       f"workload, first period from rest without dense output, literal host "
-                      f"restatement (g++ -O3 -fopenmp), {threads} OpenMP threads, {dt:.1f} s",
+                      f"restatement ({build}), {threads} OpenMP threads, {dt:.1f} s",
             "host_cpu": cpu["model"], "host_nproc": cpu["nproc"],
             "host_usable_cpus": cpu["usable_cpus"],
-            "threads_note": "16 threads = the host CPU share of one GPU on this machine",
+            "threads_note": f"{threads} threads: every usable CPU, at most 16 (the host CPU "
+                            f"share of one GPU)",
             "single_thread_value": len(one_sids) / dt1,
             "single_thread_sample": f"one in {max(1, sample // 64)} systems of that sample "
                                     f"({len(one_sids)} systems), 1 thread, {dt1:.1f} s"}

@@ -81,6 +81,19 @@ def build_variant(plant, out_path):
                     f"OUT={out_path}"], check=True)
 
 
+NATIVE_LIB_PATH = os.path.join(HERE, "libbex_oracle_native.so")
+
+
+def build_native():
+    """Builds the oracle for this host's own instruction set (-march=native), for timing the
+    CPU baseline on the machine it runs on. Returns the library path, or None if the build
+    fails."""
+    # -B: always rebuilt, so a copy built on another machine is never loaded here.
+    r = subprocess.run(["make", "-s", "-B", "-C", HERE, "native"], capture_output=True,
+                       timeout=300)
+    return NATIVE_LIB_PATH if r.returncode == 0 and os.path.exists(NATIVE_LIB_PATH) else None
+
+
 _lib = None
 
 
