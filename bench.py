@@ -38,6 +38,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak, dense (vector and matrix pipes alike)
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E peak bandwidth
 DENSE_POINTS = 64        # dense-output points per period of the headline workload
 KERNEL_NAME = "bexk::solve_kernel<RolePairLayout32<kPlain>, RKCK45, dense>"
 
@@ -481,25 +482,40 @@ def run_config4_leg(end_time, ns=4096):
     sv = pool.solver(0)
     sv.host_view(V.TimeDomain)[ens.NS:] = end_time
     sv.SynchroniseFromHostToDevice(V.All)
-    sv.reset_stats()
+    sv.checkpoint_save()
     sv.enable_timeline(True)
-    pool.synchronise()
-    t0 = time.perf_counter()
-    pool.Solve()
-    wait_with_progress(pool, [sv], "N = 256")
-    wall = time.perf_counter() - t0
-    st = sv.stats()
+    runs = {}
+    for v, name in enumerate(pool.variants(0)):
+        pool.set_variant(0, v)
+        sv.checkpoint_restore()
+        sv.reset_stats()
+        pool.synchronise()
+        t0 = time.perf_counter()
+        pool.Solve()
+        wait_with_progress(pool, [sv], f"N = 256, {name}")
+        runs[name] = (time.perf_counter() - t0, sv.stats().copy())
     pool.close()
+    best = min(runs, key=lambda k: runs[k][0])
+    wall, st = runs[best]
+    # Matrix traffic of the BiCG products: one pass over M (N^2 doubles) per iteration, two for
+    # the right-hand side and the initial residual of every stage.
+    m_bytes = 8.0 * ens.N * ens.N * (st[:, 3].sum() + 2 * st[:, 2].sum())
     achieved = flops_fm1(st, ens.N) / wall / 1e12
     return {"workload": f"{ns} systems x N=256 at random 3D positions in a 1 mm sphere (30 um "
                         f"min separation), per-system coupling, 2-3 bar at 25-50 kHz, RKCK45 "
                         f"rtol=atol=1e-10, tau in [0, {end_time}] from rest",
             "value": ns * end_time / wall, "unit": "system-periods/s (period fraction counted)",
-            "ms": wall * 1e3, "bicg_iterations_per_rhs": iters_per_rhs(st),
+            "ms": wall * 1e3, "variant": best,
+            "ms_per_variant": {k: v[0] * 1e3 for k, v in runs.items()},
+            "bicg_iterations_per_rhs": iters_per_rhs(st),
             "accepted_steps_per_system": float(st[:, 0].mean()),
-            "roofline": {"bound": "mfma", "compute_pipe": "fp64 vector ALU",
-                         "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / FP64_PEAK_TFLOPS}}
+            "roofline": {"bound": "hbm", "achieved": m_bytes / wall / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": m_bytes / wall / 1e9 / HBM_PEAK_GBS,
+                         "note": "coupling-matrix bytes of the BiCG products (one pass over the "
+                                 "system's M per iteration, per right-hand side and per initial "
+                                 "residual) over the launch; MALL hits can lift it past HBM"},
+            "fp64": {"achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS}}
 
 
 def main():

@@ -13,7 +13,7 @@
 //   h2_i  = bf16(max(z2_i, 0))
 //   y     = b3 + sum_i bf16(w3_i) * h2_i          (FP32 accumulation)
 // bf16() rounds an FP32 value to the nearest bfloat16, ties to even. Inside the solver the two
-// hidden layers run on MFMA (v_mfma_f32_32x32x8_bf16) and the output layer on the VALU.
+// hidden layers run on MFMA (v_mfma_f32_32x32x16_bf16) and the output layer on the VALU.
 #pragma once
 
 #include <stdint.h>
@@ -53,48 +53,61 @@ BEX_MLP_HD float bex_bf16_round(float f) { return bex_bf16_value(bex_bf16_bits(f
 
 // Device-side copy of a trained network in the solver's operand layout (built on the host by
 // bex_set_warm_start_network):
-//   a1[t][lane]      : MFMA A operand of layer 1, tile t (hidden rows 32t..32t+31), 4 bf16
-//   a2[t][s][lane]   : MFMA A operand of layer 2, tile t, k-step s (inputs 8s..8s+7), 4 bf16
+//   a1[t][c]         : MFMA A operand of layer 1 for the primal-half lane c (c = 0..31) of tile t
+//                      (hidden rows 32t..32t+31), 8 bf16; the shadow half supplies zeros
+//   a2[t][s][lane]   : MFMA A operand of layer 2, tile t, k-step s, 8 bf16
 //   b2, w3 (bf16-rounded, stored as FP32), b3
 struct BexMlpDevice {
-  uint64_t a1[2][64];
-  uint64_t a2[2][8][64];
+  uint64_t a1[2][32][2];
+  uint64_t a2[2][4][64][2];
   float b2[BEX_MLP_HIDDEN];
   float w3[BEX_MLP_HIDDEN];
   float b3;
   float pad[3];
 };
 
-// MFMA operand layout of v_mfma_f32_32x32x8_bf16 (one block): lane l supplies A[l % 32][k] and
-// B[k][l % 32] for k = 4 (l / 32) + e, e = 0..3 (element e in bits 16e..16e+15), and holds
-// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in accumulator r = 0..15.
+// MFMA operand layout of v_mfma_f32_32x32x16_bf16 (one block): the k-slot (g, e), g = 0..1,
+// e = 0..7, pairs element e of lane i + 32 g of the A operand (row i) with element e of lane
+// j + 32 g of the B operand (column j); lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+// in accumulator r = 0..15.
 //
-// Layer 1 computes H1^T (hidden x units) = W1^T (hidden x inputs) * X^T; layer 2 computes
-// H2^T = W2^T * H1^T with k-step s covering hidden inputs 8s..8s+7. With this layout the
-// layer-1 accumulators r = 4 (s % 4) .. 4 (s % 4) + 3 of tile s / 4 are exactly lane l's
-// B operand of layer-2 k-step s, so activations never leave the registers.
+// Layer 1 computes H1^T (hidden x units) = W1^T (hidden x inputs) * X^T with the five features,
+// the bias input 1 and two zeros in the slots (0, 0..7) and zeros in the slots (1, 0..7).
+// Layer 2 computes H2^T = W2^T * H1^T in four k-steps; k-step s feeds element e of lane l's B
+// operand from layer-1 accumulator 8 (s % 2) + e of tile s / 2, i.e. hidden input
+// bex_mlp_k2(s, l / 32, e), so activations never leave the registers. A2 is packed with the
+// W2 rows of the same hidden inputs.
+BEX_MLP_HD int bex_mlp_k2(int s, int g, int e) {
+  const int r = 8 * (s & 1) + e;
+  return 32 * (s >> 1) + 8 * (r >> 2) + 4 * g + (r & 3);
+}
+
 inline void bex_mlp_pack_device(const float* p, BexMlpDevice* d) {
   for (int t = 0; t < 2; t++)
-    for (int lane = 0; lane < 64; lane++) {
-      const int i = 32 * t + (lane & 31);
-      uint64_t v = 0;
-      for (int e = 0; e < 4; e++) {
-        const int k = 4 * (lane >> 5) + e;
-        const float w = (k < 6) ? p[BEX_MLP_OFF_W1 + k * BEX_MLP_HIDDEN + i] : 0.0f;
-        v |= (uint64_t)bex_bf16_bits(w) << (16 * e);
-      }
-      d->a1[t][lane] = v;
-    }
-  for (int t = 0; t < 2; t++)
-    for (int s = 0; s < 8; s++)
-      for (int lane = 0; lane < 64; lane++) {
-        const int i = 32 * t + (lane & 31);
+    for (int c = 0; c < 32; c++) {
+      const int i = 32 * t + c;
+      for (int w = 0; w < 2; w++) {
         uint64_t v = 0;
         for (int e = 0; e < 4; e++) {
-          const int k = 8 * s + 4 * (lane >> 5) + e;
-          v |= (uint64_t)bex_bf16_bits(p[BEX_MLP_OFF_W2 + k * BEX_MLP_HIDDEN + i]) << (16 * e);
+          const int k = 4 * w + e;
+          const float x = (k < 6) ? p[BEX_MLP_OFF_W1 + k * BEX_MLP_HIDDEN + i] : 0.0f;
+          v |= (uint64_t)bex_bf16_bits(x) << (16 * e);
         }
-        d->a2[t][s][lane] = v;
+        d->a1[t][c][w] = v;
+      }
+    }
+  for (int t = 0; t < 2; t++)
+    for (int s = 0; s < 4; s++)
+      for (int lane = 0; lane < 64; lane++) {
+        const int i = 32 * t + (lane & 31);
+        for (int w = 0; w < 2; w++) {
+          uint64_t v = 0;
+          for (int e = 0; e < 4; e++) {
+            const int k = bex_mlp_k2(s, lane >> 5, 4 * w + e);
+            v |= (uint64_t)bex_bf16_bits(p[BEX_MLP_OFF_W2 + k * BEX_MLP_HIDDEN + i]) << (16 * e);
+          }
+          d->a2[t][s][lane][w] = v;
+        }
       }
   for (int i = 0; i < BEX_MLP_HIDDEN; i++) {
     d->b2[i] = p[BEX_MLP_OFF_B2 + i];

@@ -303,8 +303,8 @@ __device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, doubl
     lay.sync();
     vec[u] = make_double2(x1sq * p, rD * pt);
     lay.sync();
-    const double a1 = lay.mv();
-    const double a2 = lay.mtv();
+    double a1, a2;
+    lay.mv_mtv(&a1, &a2);
     const double q = fma(rD, a1, p);
     const double qt = fma(x1sq, a2, pt);
     out.iterations++;
@@ -520,6 +520,12 @@ struct GroupLayout {
                                [&](int j) { return z[j].y; });
   }
 
+  // The two products of a BiCG iteration (M z on vec[].x, M^T w on vec[].y).
+  __device__ __forceinline__ void mv_mtv(double* a1, double* a2) const {
+    *a1 = mv();
+    *a2 = mtv();
+  }
+
   template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
                                               const BexKernelArgs& a, Feat&&, bool) const {
@@ -578,10 +584,23 @@ __device__ __forceinline__ double fold_to_64(const double* red, int l, Op op) {
 // evaluates identically; system-scope synchronisation is the workgroup barrier. N <= 128 keeps
 // the system's matrix in LDS (132 KB for N = 128, one system per CU); larger matrices
 // (a 256 x 256 FP64 matrix is 512 KB) exceed the CU's LDS, so they stream M and M^T
-// (column-major, both products coalesced) from global memory through L2 and MALL. With two
-// lanes per unit, each lane runs half of a row's FMA chains (row_dot4_pair) and all four SIMDs
-// of the CU take part; the per-unit work is duplicated on the unit's lanes and the first lane
-// ("primal") writes results.
+// (column-major) from global memory through L2 and MALL. With two lanes per unit, each lane
+// runs half of a row's FMA chains (row_dot4_pair) and all four SIMDs of the CU take part; the
+// per-unit work is duplicated on the unit's lanes and the first lane ("primal") writes results.
+//
+// The two products of a BiCG iteration read M once between them (mv_mtv): M is cut into
+// kNB x kNB blocks of kBS x kBS (kNB = 8 / LPU, so that the 8 N row and column FMA chains of the
+// pair, four per row of M z and four per row of M^T w, number kNB per thread), and the blocks
+// are staged through LDS one anti-diagonal I + J = t at a time, t = 0 .. 2 kNB - 2. Each thread
+// owns one chain per block index b: threads of the first half the chain c of row b kBS + rho of
+// M z, which meets block (b, t - b) at step t, i.e. its columns in ascending order; threads of
+// the second half the chain c of column b kBS + rho of M (row of M^T w), which meets block
+// (t - b, b), its rows in ascending order. So every chain is the canonical sequential chain
+// over the indices = c (mod 4), every thread runs the same number of chain segments at every
+// step, and the four chain sums combine as (s0 + s1) + (s2 + s3) as in row_dot4. The chain
+// accumulators live in LDS ([2][4][N], indexed by row and chain), an anti-diagonal is staged
+// kSlots blocks at a time, and the matrix traffic per iteration halves against reading M and
+// M^T separately.
 template <int N, bool PS = true, int LPU = 1>
 struct WideLayout {
   static_assert(N % 32 == 0 && N >= 96 && N <= 256, "multi-wave layout: 96 to 256 units");
@@ -600,16 +619,24 @@ struct WideLayout {
   static constexpr int kItemsPerBlock = 1;
   static constexpr int kSystemsPerItem = 1;
   static constexpr int kLd = N + 1;
+  static constexpr int kBS = N * LPU / 8;
+  static constexpr int kNB = 8 / LPU;
+  static constexpr int kBLd = kBS + 1;
+  static constexpr int kSlots = (LPU == 1) ? 4 : 2;
+  static_assert(kBS % 4 == 0 && kNB * kBS == N, "blocks of whole column quadruples");
+  static constexpr size_t kStageDoubles = kLdsM ? 0 : (size_t)kSlots * kBS * kBLd + 8 * N;
 
   static constexpr size_t lds_bytes() {
-    return (kLdsM ? sizeof(double) * N * (N + 1) : 0) + sizeof(double2) * N + sizeof(double) * N +
-           sizeof(Sys) + sizeof(int) * BEX_NSTAT + sizeof(int) * 4;
+    return (kLdsM ? sizeof(double) * N * (N + 1) : 0) + sizeof(double) * kStageDoubles +
+           sizeof(double2) * N + sizeof(double) * N + sizeof(Sys) + sizeof(int) * BEX_NSTAT +
+           sizeof(int) * 4;
   }
 
   int tid, u, h, sid_;
-  double* sM;         // N = 128: padded LDS copy
-  const double* gM;   // N = 256: the system's M, column-major, global memory
-  const double* gMT;  // N = 256: its transpose
+  double* sM;         // N <= 128: padded LDS copy
+  const double* gM;   // N > 128: the system's M, column-major, global memory
+  const double* gMT;  // N > 128: its transpose
+  double* stage;      // N > 128: staged blocks of M, then the chain accumulators of mv_mtv
   double2* vec;
   double* red;
   Sys* sys;
@@ -623,6 +650,8 @@ struct WideLayout {
     unsigned char* p = smem;
     sM = reinterpret_cast<double*>(p);
     if (kLdsM) p += sizeof(double) * N * (N + 1);
+    stage = reinterpret_cast<double*>(p);
+    p += sizeof(double) * kStageDoubles;
     vec = reinterpret_cast<double2*>(p);
     red = reinterpret_cast<double*>(vec + N);
     sys = reinterpret_cast<Sys*>(red + N);
@@ -715,6 +744,110 @@ struct WideLayout {
     return row([&](int j) { return mt[i + (size_t)j * N]; }, [&](int j) { return z[j].y; });
   }
 
+  // Sub-steps of the anti-diagonal pass: anti-diagonal t, staged row blocks i0 .. i1.
+  __device__ __forceinline__ static int diag_lo(int t) { return t < kNB ? 0 : t - kNB + 1; }
+  __device__ __forceinline__ static int diag_hi(int t) { return t < kNB ? t : kNB - 1; }
+  static constexpr int kPer = kBS * kBS / 2;  // double2 per block
+  static constexpr int kLoads = (kSlots * kPer + kBlock - 1) / kBlock;
+
+  // Issues this thread's loads of the blocks (i, t - i), i = i0 .. i1, into registers. Slots
+  // past the staged blocks load the last element again (same address as another lane's load)
+  // instead of branching, so that all loads are in flight together.
+  __device__ __forceinline__ void stage_load(int t, int i0, int i1, double2* v) const {
+    const int n = (i1 - i0 + 1) * kPer;
+#pragma unroll
+    for (int r = 0; r < kLoads; r++) {
+      const int q = min(tid + r * kBlock, n - 1);
+      const int sl = q / kPer;
+      const int e = q - sl * kPer;
+      const int col = e / (kBS / 2);
+      const int row = 2 * (e - col * (kBS / 2));
+      const int I = i0 + sl;
+      v[r] = *reinterpret_cast<const double2*>(gM + (size_t)((t - I) * kBS + col) * N +
+                                               I * kBS + row);
+    }
+  }
+  // Writes them to the LDS slots (padded column stride kBLd).
+  __device__ __forceinline__ void stage_store(int i0, int i1, const double2* v) const {
+    const int n = (i1 - i0 + 1) * kPer;
+#pragma unroll
+    for (int r = 0; r < kLoads; r++) {
+      const int q = tid + r * kBlock;
+      if (q < n) {
+        const int sl = q / kPer;
+        const int e = q - sl * kPer;
+        const int col = e / (kBS / 2);
+        const int row = 2 * (e - col * (kBS / 2));
+        double* d = stage + (size_t)sl * kBS * kBLd + row + kBLd * col;
+        d[0] = v[r].x;
+        d[1] = v[r].y;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void mv_mtv(double* a1, double* a2) const {
+    if (kLdsM) {
+      *a1 = mv();
+      *a2 = mtv();
+      return;
+    }
+    constexpr int kHalf = kBlock / 2;
+    const bool rows = tid < kHalf;  // chains of M z; otherwise of M^T w
+    const int k = rows ? tid : tid - kHalf;
+    const int rho = k % kBS;
+    const int c = k / kBS;
+    double* acc = stage + (size_t)kSlots * kBS * kBLd + (rows ? 0 : 4 * N) + c * N + rho;
+    for (int b = 0; b < kNB; b++) acc[b * kBS] = 0.0;
+    // The next sub-step's blocks are loaded into registers while the current one is computed.
+    double2 v[kLoads];
+    int t = 0, i0 = 0;
+    stage_load(0, 0, 0, v);
+    while (t < 2 * kNB - 1) {
+      const int i1 = min(diag_hi(t), i0 + kSlots - 1);
+      __syncthreads();  // the previous blocks are read
+      stage_store(i0, i1, v);
+      __syncthreads();
+      int tn = t, in = i1 + 1;
+      if (in > diag_hi(t)) {
+        tn = t + 1;
+        in = diag_lo(tn);
+      }
+      if (tn < 2 * kNB - 1) stage_load(tn, in, min(diag_hi(tn), in + kSlots - 1), v);
+      // Row chains meet block (b, t - b), staged for i0 <= b <= i1; column chains meet block
+      // (t - b, b), staged for i0 <= t - b <= i1.
+      const int blo = rows ? i0 : t - i1;
+      const int bhi = rows ? i1 : t - i0;
+      for (int b = blo; b <= bhi; b++) {
+        const int I = rows ? b : t - b;
+        const double* s = stage + (size_t)(I - i0) * kBS * kBLd;
+        double sum = acc[b * kBS];
+        if (rows) {
+          const double2* z = vec + (t - b) * kBS;
+#pragma unroll
+          for (int m = 0; m < kBS / 4; m++) {
+            const int j = c + 4 * m;
+            sum = fma(s[rho + kBLd * j], z[j].x, sum);
+          }
+        } else {
+          const double2* w = vec + I * kBS;
+#pragma unroll
+          for (int m = 0; m < kBS / 4; m++) {
+            const int i = c + 4 * m;
+            sum = fma(s[i + kBLd * rho], w[i].y, sum);
+          }
+        }
+        acc[b * kBS] = sum;
+      }
+      t = tn;
+      i0 = in;
+    }
+    __syncthreads();
+    const double* pm = stage + (size_t)kSlots * kBS * kBLd + u;
+    *a1 = (pm[0] + pm[N]) + (pm[2 * N] + pm[3 * N]);
+    *a2 = (pm[4 * N] + pm[5 * N]) + (pm[6 * N] + pm[7 * N]);
+    __syncthreads();  // every lane has read its sums before the accumulators are reused
+  }
+
   template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
                                               const BexKernelArgs& a, Feat&&, bool) const {
@@ -726,49 +859,68 @@ struct WideLayout {
 // RolePairLayout32: one N = 32 system per wave, primal and shadow halves.
 
 
-typedef short bex_v4s __attribute__((ext_vector_type(4)));
+typedef short bex_v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bex_v8bf __attribute__((ext_vector_type(8)));
 typedef float bex_v16f __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ bex_v4s pack_bf16x4(float a, float b, float c, float d) {
-  bex_v4s r;
-  r[0] = (short)bex_bf16_bits(a);
-  r[1] = (short)bex_bf16_bits(b);
-  r[2] = (short)bex_bf16_bits(c);
-  r[3] = (short)bex_bf16_bits(d);
-  return r;
+__device__ __forceinline__ bex_v8bf pack_bf16x8(const float* v) {
+  bex_v8s r;
+#pragma unroll
+  for (int e = 0; e < 8; e++) r[e] = (short)bex_bf16_bits(v[e]);
+  return __builtin_bit_cast(bex_v8bf, r);
+}
+__device__ __forceinline__ bex_v8bf load_bf16x8(const uint64_t* w) {
+  const uint4 q = *reinterpret_cast<const uint4*>(w);
+  return __builtin_bit_cast(bex_v8bf, q);
 }
 __device__ __forceinline__ float relu(float z) { return z > 0.0f ? z : 0.0f; }
 
 // Guess factor y of the warm-start network (definition in bex_mlp.h) for the lane's unit, on
-// both halves of a role-pair wave. The lane's half supplies inputs 4h..4h+3 of the first MFMA
-// (features 0-3 on the primal half; feature 4, the bias input 1 and two zeros on the shadow
-// half). Hidden layers run on v_mfma_f32_32x32x8_bf16 with the layer-1 accumulators feeding
-// layer 2's B operand in place; the output layer is a VALU dot over the lane's 32 hidden
-// values plus the partner half's 32.
+// both halves of a role-pair wave. The primal half supplies the layer-1 inputs (five features,
+// the bias input 1, two zeros), the shadow half zeros. Hidden layers run on
+// v_mfma_f32_32x32x16_bf16 with the layer-1 accumulators feeding layer 2's B operand in place
+// (2 + 8 MFMAs); the output layer is a VALU dot over the lane's 32 hidden values plus the
+// partner half's 32.
 __device__ __forceinline__ float mlp_factor(const BexMlpDevice* net, const double* wf, int lane) {
   const int h = lane >> 5;
-  const bex_v4s bin = (h == 0) ? pack_bf16x4((float)wf[0], (float)wf[1], (float)wf[2], (float)wf[3])
-                               : pack_bf16x4((float)wf[4], 1.0f, 0.0f, 0.0f);
+  float in[8];
+  in[0] = (float)wf[0];
+  in[1] = (float)wf[1];
+  in[2] = (float)wf[2];
+  in[3] = (float)wf[3];
+  in[4] = (float)wf[4];
+  in[5] = 1.0f;
+  in[6] = 0.0f;
+  in[7] = 0.0f;
+  if (h != 0) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) in[e] = 0.0f;
+  }
+  const bex_v8bf bin = pack_bf16x8(in);
   bex_v16f zero;
 #pragma unroll
   for (int r = 0; r < 16; r++) zero[r] = 0.0f;
+  const bex_v8bf nil = __builtin_bit_cast(bex_v8bf, make_uint4(0u, 0u, 0u, 0u));
   bex_v16f d1[2], d2[2];
 #pragma unroll
   for (int t = 0; t < 2; t++) {
-    const bex_v4s a1 = __builtin_bit_cast(bex_v4s, net->a1[t][lane]);
-    d1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, bin, zero, 0, 0, 0);
+    const bex_v8bf a1 = (h == 0) ? load_bf16x8(net->a1[t][lane & 31]) : nil;
+    d1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bin, zero, 0, 0, 0);
 #pragma unroll
     for (int r = 0; r < 16; r++) d2[t][r] = net->b2[32 * t + 8 * (r >> 2) + 4 * h + (r & 3)];
   }
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
-    const bex_v16f& src = d1[s >> 2];
-    const int q = 4 * (s & 3);
-    const bex_v4s bs = pack_bf16x4(relu(src[q]), relu(src[q + 1]), relu(src[q + 2]), relu(src[q + 3]));
+  for (int s = 0; s < 4; s++) {
+    const bex_v16f& src = d1[s >> 1];
+    const int q = 8 * (s & 1);
+    float act[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) act[e] = relu(src[q + e]);
+    const bex_v8bf bs = pack_bf16x8(act);
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      const bex_v4s a2 = __builtin_bit_cast(bex_v4s, net->a2[t][s][lane]);
-      d2[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, bs, d2[t], 0, 0, 0);
+      const bex_v8bf a2 = load_bf16x8(net->a2[t][s][lane]);
+      d2[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, bs, d2[t], 0, 0, 0);
     }
   }
   float part = 0.0f;

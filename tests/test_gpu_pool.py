@@ -156,14 +156,21 @@ def test_pool_autotune_restores_state():
 
 def test_config4_class_matches_oracle(oracle):
     """Config-4 shape: N = 256 clusters in a 1 mm sphere, per-system coupling streamed from
-    global memory; first 2 % of a period against the oracle, system by system."""
+    global memory block by block (one lane per unit: 32 x 32 blocks; two: 64 x 64); first 2 %
+    of a period against the oracle, system by system, for every launch variant."""
     classes = [wl.config4(ns=2, seed=5)]
-    print("case: config 4 class, N=256, NS=2, end 0.02", flush=True)
     pool = _pool(classes, end=0.02)
-    pool.Solve()
-    res = _results(pool, classes)
+    s = pool.solver(0)
+    s.checkpoint_save()
+    for v, name in enumerate(pool.variants(0)):
+        print(f"case: config 4 class, N=256, NS=2, end 0.02, variant {name}", flush=True)
+        pool.set_variant(0, v)
+        s.checkpoint_restore()
+        s.reset_stats()
+        pool.Solve()
+        res = _results(pool, classes)
+        _check_against_oracle(oracle, classes, res, 1, 0.02)
     pool.close()
-    _check_against_oracle(oracle, classes, res, 1, 0.02)
 
 
 def test_class_solver_solve_uses_its_own_matrices(oracle):
