@@ -1347,9 +1347,10 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::ChainLayout128<true>>("chain128 x8 waves, M in registers"),
       pool_variant<bexk::WideLayout<128, true, 2>>("wide128 lane pairs x4 waves"),
       pool_variant<bexk::WideLayout<128, true, 1>>("wide128 x2 waves")};
+  // One lane per unit first: two systems per CU keep twice the matrix loads in flight.
   static const std::vector<PoolVariant> v256 = {
-      pool_variant<bexk::WideLayout<256, true, 2>>("wide256 lane pairs x8 waves, M streamed"),
-      pool_variant<bexk::WideLayout<256, true, 1>>("wide256 x4 waves, M streamed")};
+      pool_variant<bexk::WideLayout<256, true, 1>>("wide256 x4 waves, M streamed"),
+      pool_variant<bexk::WideLayout<256, true, 2>>("wide256 lane pairs x8 waves, M streamed")};
   static const std::vector<PoolVariant> none;
   switch (n) {
     case 8: return v8;
