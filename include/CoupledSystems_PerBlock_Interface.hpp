@@ -3,9 +3,9 @@
 // Same class template, enumerations, member functions, overloads, argument meaning and error
 // behaviour as the reference's host interface
 // (SourceCodes/CoupledSystems_PerBlock_Interface.cuh:33-248): an application written against
-// the reference's ProblemSolver compiles against this header unchanged (with Precision =
-// double) and links against libbex.so, whose kernel is built for the system definition the
-// application would otherwise #include before the interface.
+// the reference's ProblemSolver compiles against this header unchanged and links against
+// libbex.so, whose kernel is built for the system definition the application would otherwise
+// #include before the interface. The integration is FP64 for Precision = float too.
 //
 // Errors the reference treats as fatal (bound checks, wrong overload/variable combination,
 // HIP failures, memory shortfall) print a message to std::cerr and exit(EXIT_FAILURE), as in
@@ -341,8 +341,12 @@ template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, i
           int NGP, int NiGP, int NUA, int NiUA, int NSA, int NiSA, int NE, int NDO,
           Algorithms Algorithm, class Precision>
 class ProblemSolver {
-  static_assert(std::is_same<Precision, double>::value,
-                "the MI355X solver integrates in FP64 (Precision = double)");
+  // The reference instantiates Precision = float or double (Interface.cuh:135). Both are
+  // accepted here; the solver integrates in FP64 either way (collapse stiffness needs it), so
+  // with float the values are converted at SetHost/GetHost and the results are the FP64 solve
+  // of the float inputs rather than a single-precision solve.
+  static_assert(std::is_same<Precision, double>::value || std::is_same<Precision, float>::value,
+                "Precision must be float or double");
 
  private:
   bex_solver* handle_ = nullptr;

@@ -166,3 +166,24 @@ def test_cpp_header_writers_follow_the_format_rules(tmp_path):
     assert (tmp_path / "i.txt").read_text() == EXPECTED_INT
     assert (tmp_path / "s.txt").read_text() == EXPECTED_SYSTEM
     assert (tmp_path / "d.txt").read_text() == EXPECTED_DENSE
+
+
+FLOAT_INSTANCE = r"""
+#include "CoupledSystems_PerBlock_Interface.hpp"
+// The reference's float instantiation compiles; Set/Get take and return float.
+using Solver = ProblemSolver<4, 8, 2, 8, 1, 1, 0, 0, 21, 0, 0, 0, 64, 1, 0, 0, 1, 0, RKCK45, float>;
+float probe(Solver& s) {
+  s.SetHost(0, 0, ActualState, 0, 1.0f);
+  s.SolverOption(RelativeTolerance, 0, 1e-6f);
+  return s.GetHost<float>(0, 0, ActualState, 0);
+}
+"""
+
+
+def test_cpp_header_accepts_float_precision(tmp_path):
+    if not shutil.which("g++"):
+        pytest.skip("no host compiler")
+    src = tmp_path / "float_instance.cpp"
+    src.write_text(FLOAT_INSTANCE)
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(REPO, "include"),
+                    str(src)], check=True)
