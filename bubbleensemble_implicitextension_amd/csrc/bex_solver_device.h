@@ -274,13 +274,12 @@ __device__ __forceinline__ BicgOut group_coupling(const L& lay, double* F, doubl
   double2* vec = lay.vec;
   const int u = lay.uid();
   lay.sync();
-  vec[u].x = cpt;
+  vec[u] = make_double2(cpt, x1sq * xu);
   lay.sync();
-  const double b = fma(rD, lay.mv(), xu);
-  lay.sync();
-  vec[u].x = x1sq * xu;
-  lay.sync();
-  const double ax = fma(rD, lay.mv(), xu);
+  double mc, mx;
+  lay.mv_two(&mc, &mx);  // M CPT and M (x1^2 x0)
+  const double b = fma(rD, mc, xu);
+  const double ax = fma(rD, mx, xu);
   double r = -ax + b;
   double rt = r;
   double x = xu, p = 0.0, pt = 0.0, rho2 = 0.0;
@@ -525,6 +524,15 @@ struct GroupLayout {
     *a1 = mv();
     *a2 = mtv();
   }
+  // M z1 and M z2 for z1 on vec[].x and z2 on vec[].y.
+  __device__ __forceinline__ void mv_two(double* a1, double* a2) const {
+    const double* m = sM;
+    const double2* z = vec;
+    const int i = uid_;
+    row_dot4_two<N, kChunk>([&](int j) { return m[i + j * kLd]; },
+                            [&](int j) { return z[j].x; }, [&](int j) { return z[j].y; }, a1,
+                            a2);
+  }
 
   template <class Feat>
   __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
@@ -560,6 +568,54 @@ __device__ __forceinline__ double row_dot4_pair(MF m, ZF z, int h) {
   // quad_perm [1, 0, 3, 2]: the partner lane of the unit
   const double other = __builtin_amdgcn_mov_dpp(pair, 0xB1, 0xf, 0xf, false);
   return h == 0 ? pair + other : other + pair;
+}
+
+// Two row products over the same matrix row, m(j) z1(j) and m(j) z2(j), each in row_dot4's
+// canonical order (bit-identical to two row_dot4 calls), reading each m(j) once.
+template <int N, int C, class MF, class ZF1, class ZF2>
+__device__ __forceinline__ void row_dot4_two(MF m, ZF1 z1, ZF2 z2, double* r1, double* r2) {
+  static_assert(C % 4 == 0 && N % C == 0, "chunk of whole column quadruples");
+  double s[4] = {0.0, 0.0, 0.0, 0.0}, t[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int j0 = 0; j0 < N; j0 += C) {
+    double mm[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) mm[k] = m(j0 + k);
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      s[k & 3] = fma(mm[k], z1(j0 + k), s[k & 3]);
+      t[k & 3] = fma(mm[k], z2(j0 + k), t[k & 3]);
+    }
+  }
+  *r1 = (s[0] + s[1]) + (s[2] + s[3]);
+  *r2 = (t[0] + t[1]) + (t[2] + t[3]);
+}
+
+// The same for two lanes per unit (as row_dot4_pair).
+template <int N, int C, class MF, class ZF1, class ZF2>
+__device__ __forceinline__ void row_dot4_pair_two(MF m, ZF1 z1, ZF2 z2, int h, double* r1,
+                                                  double* r2) {
+  static_assert(C % 4 == 0 && N % C == 0, "chunk of whole column quadruples");
+  double sa = 0.0, sb = 0.0, ta = 0.0, tb = 0.0;
+#pragma unroll 1
+  for (int j0 = 0; j0 < N; j0 += C) {
+    double mm[C / 2];
+#pragma unroll
+    for (int k = 0; k < C / 2; k++) mm[k] = m(j0 + 4 * (k >> 1) + 2 * h + (k & 1));
+#pragma unroll
+    for (int k = 0; k < C / 2; k += 2) {
+      const int j = j0 + 4 * (k >> 1) + 2 * h;
+      sa = fma(mm[k], z1(j), sa);
+      sb = fma(mm[k + 1], z1(j + 1), sb);
+      ta = fma(mm[k], z2(j), ta);
+      tb = fma(mm[k + 1], z2(j + 1), tb);
+    }
+  }
+  const double p1 = sa + sb, p2 = ta + tb;
+  const double o1 = __builtin_amdgcn_mov_dpp(p1, 0xB1, 0xf, 0xf, false);
+  const double o2 = __builtin_amdgcn_mov_dpp(p2, 0xB1, 0xf, 0xf, false);
+  *r1 = h == 0 ? p1 + o1 : o1 + p1;
+  *r2 = h == 0 ? p2 + o2 : o2 + p2;
 }
 
 // Levels of the reference's reduction fold (Steppers.cuh:745-805) above 64 entries, for N a
@@ -742,6 +798,24 @@ struct WideLayout {
     }
     const double* mt = gMT;
     return row([&](int j) { return mt[i + (size_t)j * N]; }, [&](int j) { return z[j].y; });
+  }
+  // M z1 and M z2 for z1 on vec[].x and z2 on vec[].y, each row of M read once.
+  __device__ __forceinline__ void mv_two(double* a1, double* a2) const {
+    const double2* z = vec;
+    const int i = u;
+    auto z1 = [&](int j) { return z[j].x; };
+    auto z2 = [&](int j) { return z[j].y; };
+    if (kLdsM) {
+      const double* m = sM;
+      auto mf = [&](int j) { return m[i + j * kLd]; };
+      if (LPU == 2) row_dot4_pair_two<N, kChunk>(mf, z1, z2, h, a1, a2);
+      else row_dot4_two<N, kChunk>(mf, z1, z2, a1, a2);
+      return;
+    }
+    const double* m = gM;
+    auto mf = [&](int j) { return m[i + (size_t)j * N]; };
+    if (LPU == 2) row_dot4_pair_two<N, kChunk>(mf, z1, z2, h, a1, a2);
+    else row_dot4_two<N, kChunk>(mf, z1, z2, a1, a2);
   }
 
   // Sub-steps of the anti-diagonal pass: anti-diagonal t, staged row blocks i0 .. i1.
