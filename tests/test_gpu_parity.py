@@ -11,6 +11,7 @@ import pytest
 
 from bubbleensemble_implicitextension_amd import workloads as wl
 from bubbleensemble_implicitextension_amd.problem_solver import LAYOUT_GROUP, LAYOUT_ROLE_PAIR
+from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
 from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
 
@@ -74,6 +75,33 @@ def test_config0_two_periods_with_reset(oracle, layout):
     compare_solver_to_oracle(s, p)
     st = s.stats()
     assert st[:, 6].sum() > 0, "no collapse events were recorded"
+
+
+def test_config0_eight_periods_digests():
+    """Config 0 without its singular corner over the collapse-strength protocol's shape (4
+    transient periods, accessory reset, 4 recording periods, 62600 recorded maxima): every
+    system's state, time, accessories and counters equal the oracle's, compared through the
+    per-system digests the oracle wrote (tests/golden/make_config0_periods.py)."""
+    import json
+    import os
+    import sys
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config0_periods.json")))
+    ens = wl.config0().subset(want["systems"])
+    print(f"case: config 0, {ens.NS} systems, {want['transient']}+{want['record']} periods",
+          flush=True)
+    s = run_gpu(ens, periods=want["transient"] + want["record"], reset_after=want["transient"])
+    got = system_digests(ens.NS, ens.N, s.host_view(V.ActualState), s.host_view(V.ActualTime),
+                         s.host_view(V.UnitAccessories), s.host_view(V.IntegerUnitAccessories),
+                         s.stats())
+    bad = [want["systems"][k] for k in range(ens.NS) if got[k] != want["digests"][k]]
+    assert not bad, f"systems differing from the oracle: {bad}"
+    assert int(s.event_counts().sum()) == want["events_total"]
+    s.close()
 
 
 def test_singular_corner_short_window(oracle):

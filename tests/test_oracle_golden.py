@@ -68,3 +68,29 @@ def test_canonical_and_literal_modes_agree(oracle):
     acc_c = can.accessories()[stable]
     acc_l = lit.accessories()[stable]
     np.testing.assert_allclose(acc_l, acc_c, rtol=2e-5, atol=1e-12)
+
+
+def test_config0_period_digests_reproduce(oracle):
+    """The committed per-system digests of config 0 over 4 + 4 periods (the GPU test's
+    reference) are the oracle's: recomputed here for the six cheapest systems (200 kHz)."""
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config0_periods.json")))
+    pick = [k for k, sid in enumerate(want["systems"]) if sid >= 58]
+    ens = wl.config0().subset([want["systems"][k] for k in pick])
+    p = oracle.OracleProblem.from_ensemble(ens)
+    for k in range(want["transient"] + want["record"]):
+        if k == want["transient"]:
+            p.reset_accessories()
+        p.solve(mode=oracle.CANONICAL)
+    got = system_digests(ens.NS, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
+                         p.IntegerUnitAccessories, p.stats_table())
+    assert got == [want["digests"][k] for k in pick]
