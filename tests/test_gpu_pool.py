@@ -173,6 +173,33 @@ def test_config4_class_matches_oracle(oracle):
     pool.close()
 
 
+def test_config4_sixteen_systems_digests():
+    """Sixteen config-4 systems (N = 256, per-system coupling streamed from global memory) over
+    the first tenth of a period, one pool class: every system's state, time, accessories and
+    counters equal the oracle's, through the per-system digests the oracle wrote
+    (tests/golden/make_config4_digests.py)."""
+    import json
+    import os
+    import sys
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config4_digests.json")))
+    ens = wl.config4(ns=want["ns"], seed=want["seed"])
+    print(f"case: config 4, {ens.NS} systems x N=256, tau in [0, {want['end_time']}]", flush=True)
+    pool = _pool([ens], end=want["end_time"])
+    pool.Solve()
+    res = _results(pool, [ens])[0]
+    pool.close()
+    got = system_digests(ens.NS, ens.N, res["state"], res["time"], res["acc"], res["iacc"],
+                         res["stats"])
+    bad = [k for k in range(ens.NS) if got[k] != want["digests"][k]]
+    assert not bad, f"systems differing from the oracle: {bad}"
+    assert int((res["iacc"] + 1).sum()) == want["events_total"]
+
+
 def test_class_solver_solve_uses_its_own_matrices(oracle):
     """Solve() on a pool class's own solver runs that class alone with each system's matrix
     (not the first system's): every system matches the oracle."""

@@ -94,3 +94,27 @@ def test_config0_period_digests_reproduce(oracle):
     got = system_digests(ens.NS, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
                          p.IntegerUnitAccessories, p.stats_table())
     assert got == [want["digests"][k] for k in pick]
+
+
+def test_config4_digest_reproduces(oracle):
+    """The committed config-4 digests (the GPU pool test's reference) are the oracle's:
+    recomputed here for the first system."""
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config4_digests.json")))
+    ens = wl.config4(ns=want["ns"], seed=want["seed"]).subset([0])
+    ens.coupling = ens.coupling[0]
+    p = oracle.OracleProblem.from_ensemble(ens)
+    p.TimeDomain[1:] = want["end_time"]
+    p.solve(mode=oracle.CANONICAL, nthreads=1)
+    got = system_digests(1, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
+                         p.IntegerUnitAccessories, p.stats_table())
+    assert got[0] == want["digests"][0]
