@@ -200,6 +200,43 @@ def test_config4_sixteen_systems_digests():
     assert int((res["iacc"] + 1).sum()) == want["events_total"]
 
 
+def test_config2_five_classes_digests():
+    """Sixteen systems of each config-2 size class (N = 8 ... 128), one pool launch over the
+    first quarter of a period (autotuned variants and queue order): every system's state, time,
+    accessories and counters equal the oracle's, through the per-system digests the oracle
+    wrote (tests/golden/make_config2_digests.py)."""
+    import json
+    import os
+    import sys
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config2_digests.json")))
+    classes = [wl.config2_class(n, want["ns"], want["seed"]) for n in want["units"]]
+    print(f"case: config 2, {len(classes)} classes x {want['ns']} systems, "
+          f"tau in [0, {want['end_time']}]", flush=True)
+    pool = _pool(classes, end=want["end_time"])
+    for c, e in enumerate(classes):
+        pool.solver(c).checkpoint_save()
+    pool.autotune()
+    for c, e in enumerate(classes):
+        pool.solver(c).checkpoint_restore()
+        pool.solver(c).reset_stats()
+    pool.Solve()
+    res = _results(pool, classes)
+    pool.close()
+    events = 0
+    for c, e in enumerate(classes):
+        r = res[c]
+        got = system_digests(e.NS, e.N, r["state"], r["time"], r["acc"], r["iacc"], r["stats"])
+        bad = [k for k in range(e.NS) if got[k] != want["digests"][str(e.N)][k]]
+        assert not bad, f"N={e.N}: systems differing from the oracle: {bad}"
+        events += int((r["iacc"] + 1).sum())
+    assert events == want["events_total"]
+
+
 def test_class_solver_solve_uses_its_own_matrices(oracle):
     """Solve() on a pool class's own solver runs that class alone with each system's matrix
     (not the first system's): every system matches the oracle."""

@@ -118,3 +118,28 @@ def test_config4_digest_reproduces(oracle):
     got = system_digests(1, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
                          p.IntegerUnitAccessories, p.stats_table())
     assert got[0] == want["digests"][0]
+
+
+def test_config2_digests_reproduce(oracle):
+    """The committed config-2 digests (the GPU pool test's reference) are the oracle's:
+    recomputed here for the first system of the N = 8, 32 and 64 classes."""
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config2_digests.json")))
+    for n in (8, 32, 64):
+        ens = wl.config2_class(n, want["ns"], want["seed"]).subset([0])
+        ens.coupling = ens.coupling[0]
+        p = oracle.OracleProblem.from_ensemble(ens)
+        p.TimeDomain[1:] = want["end_time"]
+        p.solve(mode=oracle.CANONICAL, nthreads=1)
+        got = system_digests(1, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
+                             p.IntegerUnitAccessories, p.stats_table())
+        assert got[0] == want["digests"][str(n)][0], n
