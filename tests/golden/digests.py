@@ -20,3 +20,18 @@ def system_digests(NS, N, state, time, acc, iacc, stats):
             h.update(np.ascontiguousarray(part).tobytes())
         out.append(h.hexdigest())
     return out
+
+
+def dense_digests(NS, N, NDO, times, states, index, UD=2):
+    """Per-system digests of dense output: the system's NDO output times, its [NDO][UD][N]
+    states (reference layout [idx][comp][NS*N]) and its index."""
+    times = np.asarray(times, dtype=np.float64).reshape(NDO, NS)
+    states = np.asarray(states, dtype=np.float64).reshape(NDO, UD, NS, N)
+    index = np.asarray(index, dtype=np.int32).reshape(NS)
+    out = []
+    for k in range(NS):
+        h = hashlib.sha256()
+        for part in (times[:, k], states[:, :, k, :], index[k:k + 1]):
+            h.update(np.ascontiguousarray(part).tobytes())
+        out.append(h.hexdigest())
+    return out

@@ -104,6 +104,53 @@ def test_config0_eight_periods_digests():
     s.close()
 
 
+def test_config1_full_period_digests():
+    """The headline workload as the benchmark runs it — all 16384 systems of config 1, one
+    period from rest, 64 Hermite dense-output points — equals the oracle system by system:
+    state, time, accessories, counters and dense output, through per-system digests condensed
+    to 256 block digests (tests/golden/make_config1_digests.py)."""
+    import hashlib
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import collapse
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import dense_digests, system_digests
+
+    want = json.load(open(os.path.join(here, "config1_digests.json")))
+    ens = wl.config1()
+    ndo = want["ndo"]
+    print(f"case: config 1, {ens.NS} systems, one period, {ndo} dense points", flush=True)
+    s = collapse.make_solver(ens, NDO=ndo)
+    collapse.fill(s, ens)
+    s.SolverOption(O.DenseOutputInterpolation, 1)
+    s.SolverOption(O.DenseOutputMinimumTimeStep, 1.0 / ndo)
+    s.SynchroniseFromHostToDevice(V.All)
+    s.Solve()
+    s.SynchroniseFromDeviceToHost(V.All)
+    s.SynchroniseFromDeviceToHost(V.DenseOutput)
+    s.InsertSynchronisationPoint()
+    s.SynchroniseSolver()
+    d = system_digests(ens.NS, ens.N, s.host_view(V.ActualState), s.host_view(V.ActualTime),
+                       s.host_view(V.UnitAccessories), s.host_view(V.IntegerUnitAccessories),
+                       s.stats())
+    dd = dense_digests(ens.NS, ens.N, ndo, s.host_view(V.DenseTime), s.host_view(V.DenseState),
+                       s.host_view(V.DenseIndex))
+    per = [x + y for x, y in zip(d, dd)]
+    blk = want["block"]
+    got = [hashlib.sha256("".join(per[b:b + blk]).encode()).hexdigest()
+           for b in range(0, ens.NS, blk)]
+    bad = [b for b in range(len(got)) if got[b] != want["blocks"][b]]
+    assert not bad, f"blocks of {blk} systems differing from the oracle: {bad}"
+    assert int(s.event_counts().sum()) == want["events_total"]
+    assert int(s.stats()[:, 0].sum()) == want["accepted_total"]
+    s.close()
+
+
 def test_singular_corner_short_window(oracle):
     """The Mach-1 corner system over the first 5 % of a period (golden case cfg0_corner)."""
     ens = wl.config0().subset([7])

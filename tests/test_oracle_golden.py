@@ -143,3 +143,32 @@ def test_config2_digests_reproduce(oracle):
         got = system_digests(1, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
                              p.IntegerUnitAccessories, p.stats_table())
         assert got[0] == want["digests"][str(n)][0], n
+
+
+def test_config1_block_digest_reproduces(oracle):
+    """The committed config-1 block digests (the full-size GPU test's reference) are the
+    oracle's: the last block of 64 systems (200 kHz) recomputed here, dense output included."""
+    import hashlib
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import dense_digests, system_digests
+
+    want = json.load(open(os.path.join(here, "config1_digests.json")))
+    ndo, blk = want["ndo"], want["block"]
+    last = len(want["blocks"]) - 1
+    ens = wl.config1().subset(range(last * blk, (last + 1) * blk))
+    p = oracle.OracleProblem.from_ensemble(ens, NDO=ndo)
+    p.opts.update({"DenseOutputInterpolation": 1, "DenseOutputMinimumTimeStep": 1.0 / ndo})
+    p.solve(mode=oracle.CANONICAL)
+    d = system_digests(ens.NS, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
+                       p.IntegerUnitAccessories, p.stats_table())
+    dd = dense_digests(ens.NS, ens.N, ndo, p.DenseOutputTimeInstances, p.DenseOutputStates,
+                       p.DenseOutputIndex)
+    got = hashlib.sha256("".join(x + y for x, y in zip(d, dd)).encode()).hexdigest()
+    assert got == want["blocks"][last]
