@@ -599,9 +599,11 @@ def main():
     log(f"plain leg done: {wall * 1e3 / K:.1f} ms per period")
 
     line = {
-        "metric": "systems x acoustic periods/s at rtol=atol=1e-10, plain FP64 RKCK45 + BiCG "
-                  "(learned warm start vs plain in 'learned', learned+batched in "
-                  "'learned_batched'); BiCG iters/RHS",
+        "metric": "systems\u00d7acoustic periods/sec, learned vs plain at rtol 1e-10; BiCG "
+                  "iters/RHS",
+        "metric_note": "value: the plain FP64 RKCK45 + BiCG method at rtol=atol=1e-10; the "
+                       "learned warm start in 'learned', learned+batched in 'learned_batched', "
+                       "BiCG iterations per RHS in 'bicg_iterations_per_rhs'",
         "value": value,
         "unit": "system-periods/s",
         "n_gpus": 1,
