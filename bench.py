@@ -336,12 +336,14 @@ def wait_with_progress(pool, solvers, label, every=20.0, limit=None):
     return finished_at_stop
 
 
-def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None):
+def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None, cap=0):
     """Config 2 (20480 systems, N in {8, 16, 32, 64, 128}, per-system coupling) through a
     ProblemPool: autotuned variants and queue order, one graph launch over [0, end_time] from
-    rest, then (optionally) the same classes launched one after another."""
+    rest, then (optionally) the same classes launched one after another. cap > 0 ends every
+    system after that many step attempts (MaximumStepAttempts)."""
     from bubbleensemble_implicitextension_amd import collapse
     from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
     from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
     classes = wl.config2()
@@ -351,6 +353,8 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None):
         for c, e in enumerate(cls):
             sv = pl.solver(c)
             collapse.fill(sv, e)
+            if cap:
+                sv.SolverOption(O.MaximumStepAttempts, cap)
             sv.host_view(V.TimeDomain)[e.NS:] = end_time
             sv.SynchroniseFromHostToDevice(V.All)
             sv.reset_stats()
@@ -408,6 +412,8 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None):
         "bicg_iterations_per_rhs": {str(e.N): iters_per_rhs(st) for st, e in zip(stats, classes)},
         "class_last_system_end_s": slowest,
         "systems_completed": completed,
+        "attempt_cap": cap or None,
+        "systems_capped_or_stopped": {str(e.N): e.NS - completed[str(e.N)] for e in classes},
         "stopped_after_s": None if stopped_after is None else limit,
         "forced_accepts_per_system_max": {str(e.N): int(st[:, 7].max())
                                           for st, e in zip(stats, classes)},
@@ -539,6 +545,8 @@ def main():
     ap.add_argument("--leg-time-limit", type=float, default=None,
                     help="with --leg: seconds after which the leg's remaining systems are asked "
                          "to stop (their state is written back as it stands)")
+    ap.add_argument("--attempt-cap", type=int, default=0,
+                    help="with --leg pool: MaximumStepAttempts of every class (0: no cap)")
     ap.add_argument("--config4-end-time", type=float, default=0.02,
                     help="period fraction of the N = 256 leg (0 skips it)")
     ap.add_argument("--end-time", type=float, default=1.0,
@@ -555,7 +563,8 @@ def main():
         return copy_calibration()
     if args.leg:
         log(f"{args.leg} leg alone")
-        rec = (run_pool_leg(args.pool_end_time, sequential=False, limit=args.leg_time_limit)
+        rec = (run_pool_leg(args.pool_end_time, sequential=False, limit=args.leg_time_limit,
+                            cap=args.attempt_cap)
                if args.leg == "pool"
                else run_config4_leg(args.config4_end_time) if args.leg == "n256"
                else run_second_period_leg(limit=args.leg_time_limit))

@@ -74,6 +74,7 @@ struct bex_solver {
   int BiCGMaxIterations = 19;
   int KernelLayout = BEX_LAYOUT_AUTO;
   int PriorityStepThreshold = 3000;
+  long long MaxStepAttempts = 0;  // 0: no cap
   int WarmStart = 0;                          // 1: learned BiCG initial guess
   BexMlpDevice* d_net = nullptr;              // network in the kernel's operand layout
   float* d_samples = nullptr;                 // [SampleCapacity][BEX_SAMPLE_FLOATS] samples
@@ -239,6 +240,7 @@ void fill_args(const bex_solver* s, BexKernelArgs* ap) {
   a.Stats = s->d_stats;
   a.Timeline = s->d_timeline;
   a.StopFlag = s->d_stop;
+  a.MaxStepAttempts = s->MaxStepAttempts;
   a.InitialTimeStep = s->InitialTimeStep;
   a.MaximumTimeStep = s->MaximumTimeStep;
   a.MinimumTimeStep = s->MinimumTimeStep;
@@ -658,6 +660,11 @@ int bex_solver_option(bex_solver* s, int option, double v) {
     case BEX_BiCGResidualTolerance: s->BiCGResidualTolerance = v; break;
     case BEX_BiCGMaxIterations: s->BiCGMaxIterations = (int)v; break;
     case BEX_PriorityStepThreshold: s->PriorityStepThreshold = (int)v; break;
+    case BEX_MaximumStepAttempts:
+      if (!(v >= 0.0) || v > 9.0e18)
+        return fail(s, BEX_E_ARGUMENT, "SolverOption: MaximumStepAttempts must be >= 0");
+      s->MaxStepAttempts = (long long)v;
+      break;
     case BEX_KernelLayout: {
       KernelChoice kc;
       if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, bexk::kDenseNone, &kc))

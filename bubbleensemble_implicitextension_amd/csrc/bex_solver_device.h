@@ -1995,14 +1995,18 @@ __device__ __forceinline__ void integrate(L& lay, const BexKernelArgs& a) {
   // takes its next system from a queue starts it at the base level again.
   __builtin_amdgcn_s_setprio(0);
   long long stage_count = 0;  // RK stages of this launch (sample selection)
-  int polls = 0;
+  long long polls = 0;
   while (terminate == 0) {  // Solver.cuh:329-572
-    // Cooperative stop (no reference counterpart): every 256 attempts the system reads the
-    // host's stop flag; a set flag ends the system here, as a user termination would, with its
-    // state and time written back as they stand. The decision is taken for the whole system.
-    if (a.StopFlag != nullptr && (++polls & 255) == 0) {
-      const int req = *reinterpret_cast<const volatile int*>(a.StopFlag);
-      if (lay.any(req != 0)) {
+    // Cooperative stop and attempt cap (no reference counterpart): every 256 attempts the
+    // system reads the host's stop flag and compares its attempts in this launch with
+    // MaxStepAttempts; a set flag or a reached cap ends the system here, as a user termination
+    // would, with its state and time written back as they stand. Both decisions are taken for
+    // the whole system (the attempt count is the same on every lane).
+    if ((++polls & 255) == 0) {
+      const bool capped = a.MaxStepAttempts > 0 && polls >= a.MaxStepAttempts;
+      const int req =
+          a.StopFlag != nullptr ? *reinterpret_cast<const volatile int*>(a.StopFlag) : 0;
+      if (capped || lay.any(req != 0)) {
         terminate = 1;
         continue;
       }

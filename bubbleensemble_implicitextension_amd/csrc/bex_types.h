@@ -59,6 +59,9 @@ struct BexKernelArgs {
   // Host-mapped stop flag: while it is nonzero, every running system ends at its next check
   // (every 256 step attempts) with its state and time written back as they stand.
   const int* StopFlag;
+  // > 0: a system ends at its first check (every 256 step attempts) after this many attempts in
+  // the launch, state and time written back as they stand.
+  long long MaxStepAttempts;
 
   double InitialTimeStep;
   double MaximumTimeStep;

@@ -65,6 +65,7 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     PriorityStepThreshold = 18
     WarmStart = 19
     DenseOutputInterpolation = 20
+    MaximumStepAttempts = 21
 
 
 LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR = 0, 1, 2

@@ -68,7 +68,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   KernelLayout,
   PriorityStepThreshold,
   WarmStart,
-  DenseOutputInterpolation
+  DenseOutputInterpolation,
+  MaximumStepAttempts
 };
 
 // --- Names of the enumerators (Interface.cuh:70-71, 2035-2106) ----------------------------
@@ -81,7 +82,8 @@ inline std::string SolverOptionsToString(ListOfSolverOptions Option) {
       "EventDirection",        "DenseOutputSaveFrequency", "DenseOutputMinimumTimeStep",
       "SharedGlobalVariables", "SharedCouplingMatrices", "BiCGRhoTolerance",
       "BiCGResidualTolerance", "BiCGMaxIterations",      "KernelLayout",
-      "PriorityStepThreshold", "WarmStart",              "DenseOutputInterpolation"};
+      "PriorityStepThreshold", "WarmStart",              "DenseOutputInterpolation",
+      "MaximumStepAttempts"};
   const int i = static_cast<int>(Option) - static_cast<int>(InitialTimeStep);
   const int n = static_cast<int>(sizeof(names) / sizeof(names[0]));
   return (i >= 0 && i < n) ? names[i] : "Non-existent solver option!";

@@ -92,7 +92,13 @@ enum bex_option {
    * the accepted step containing each instant (from its end points and their derivatives),
    * instead of the reference's step-point sampling (DenseOutput.cuh:313-395); the index still
    * persists across Solve() calls. No reference counterpart. */
-  BEX_DenseOutputInterpolation
+  BEX_DenseOutputInterpolation,
+  /* > 0: a system that has made this many step attempts in one Solve() ends at its next check
+   * (checks every 256 attempts), with its state and time written back as they stand; 0 (the
+   * default) runs every system to the end of its time domain as the reference does. Bounds a
+   * launch whose systems reach the Keller-Miksis Mach-1 singularity (millions of forced steps
+   * at the minimum step size). No reference counterpart. */
+  BEX_MaximumStepAttempts
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32, 64 and 128 units, lane groups otherwise */

@@ -166,3 +166,38 @@ def test_stop_request_ends_systems_at_their_next_check():
     assert np.all(np.abs(s.host_view(V.ActualTime) - 0.3) < 1e-12)
     assert np.all(s.stats()[stopped, 10] > 255)
     s.close()
+
+
+def test_attempt_cap_ends_long_systems_only():
+    """MaximumStepAttempts = 1100: a system still running at its first check past the cap
+    (the fifth, after 1279 attempts) ends there with its state and time as they stand; systems
+    that finish sooner are bit-identical to an uncapped solve. Negative caps are rejected."""
+    ens = wl.config0().subset([9, 18, 27, 36, 0])
+    print(f"case: attempt cap 1100, {ens.NS} systems, tau in [0, 0.3]", flush=True)
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.host_view(V.TimeDomain)[ens.NS:] = 0.3
+    s.SynchroniseFromHostToDevice(V.All)
+    s.checkpoint_save()
+    s.Solve()
+    free = _snapshot(s)
+    free_stats = s.stats().copy()
+    attempts = free_stats[:, 10]
+    s.reset_stats()
+    s.checkpoint_restore()
+    s.SolverOption(O.MaximumStepAttempts, 1100)
+    s.Solve()
+    capped = _snapshot(s)
+    st = s.stats()
+    long = attempts >= 1280
+    assert long.any() and (~long).any(), attempts
+    assert np.all(st[long, 10] == 1279), st[:, 10]
+    assert np.all(capped[V.ActualTime][long] < 0.3 - 1e-9)
+    state_free = free[V.ActualState].reshape(2, ens.NS, ens.N)
+    state_capped = capped[V.ActualState].reshape(2, ens.NS, ens.N)
+    assert_bitwise("uncapped systems", state_capped[:, ~long], state_free[:, ~long])
+    assert_bitwise("uncapped stats", st[~long], free_stats[~long])
+    with pytest.raises(BexError) as e:
+        s.SolverOption(O.MaximumStepAttempts, -1)
+    assert e.value.code == BEX_E_ARGUMENT
+    s.close()
