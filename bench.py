@@ -16,8 +16,9 @@ copied to the host is reported beside it).
 
 Further legs, each skipped (and said so) when the wall budget would not hold it: the learned
 BiCG warm start, learned + batched, the BiCG residual 1e-13 workload (plain and learned), the
-uneven-N pool (20480 systems, a tenth of a period) and the N = 256 clusters (4096 systems, a
-fraction of a period). The CPU baseline times the host restatement on a bounded sample.
+second period of the headline workload (the steady state, with an attempt cap for the systems
+that reach the Keller-Miksis Mach-1 singularity), the uneven-N pool (20480 systems, a tenth of
+a period) and the N = 256 clusters (4096 systems, a fraction of a period). The CPU baseline times the host restatement on a bounded sample.
 
 Prints one JSON line (metric, value = whole-job system-periods/s of the plain method, roofline
 of the solver kernel on the FP64 pipeline, CPU baseline).
@@ -431,11 +432,13 @@ def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None, cap
     return out
 
 
-def run_second_period_leg(limit=None):
+def run_second_period_leg(limit=None, cap=0):
     """Headline workload's second period (continuing from the end of the first, no dense
     output): per-system finish times from the kernel timeline, so the launch's tail (the systems
-    near the Keller-Miksis Mach-1 singularity) and its bulk are reported apart."""
+    near the Keller-Miksis Mach-1 singularity) and its bulk are reported apart. cap > 0 ends a
+    system after that many step attempts of the second period (MaximumStepAttempts)."""
     from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
     from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
     ens = wl.config1()
@@ -444,6 +447,8 @@ def run_second_period_leg(limit=None):
     s.SynchroniseDevice()
     first = s.stats()
     s.reset_stats()
+    if cap:
+        s.SolverOption(O.MaximumStepAttempts, cap)
     s.enable_timeline(True)
     s.SynchroniseDevice()
     t0 = time.perf_counter()
@@ -462,8 +467,12 @@ def run_second_period_leg(limit=None):
     completed = int(np.count_nonzero(s.host_view(V.ActualTime) >= 1.0 - 1e-9))
     s.close()
     order = np.argsort(-end)
+    achieved = flops_fm1(st, ens.N) / wall / 1e12
     return {"workload": "16384 systems x N=32, second period (continuing from the first)",
-            "wall_s": wall, "systems_completed": completed,
+            "value": ens.NS / wall, "unit": "system-periods/s",
+            "wall_s": wall, "systems_completed": completed, "attempt_cap": cap or None,
+            "fp64": {"achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS},
             "stopped_after_s": None if stopped_after is None else limit, "finish_s_quantiles": {q: float(np.quantile(end, float(q)))
                                                    for q in ("0.5", "0.9", "0.99", "0.999")},
             "last_systems": [{"system": int(k), "finish_s": float(end[k]),
@@ -546,7 +555,7 @@ def main():
                     help="with --leg: seconds after which the leg's remaining systems are asked "
                          "to stop (their state is written back as it stands)")
     ap.add_argument("--attempt-cap", type=int, default=0,
-                    help="with --leg pool: MaximumStepAttempts of every class (0: no cap)")
+                    help="with --leg pool or period2: MaximumStepAttempts (0: no cap)")
     ap.add_argument("--config4-end-time", type=float, default=0.02,
                     help="period fraction of the N = 256 leg (0 skips it)")
     ap.add_argument("--end-time", type=float, default=1.0,
@@ -567,7 +576,7 @@ def main():
                             cap=args.attempt_cap)
                if args.leg == "pool"
                else run_config4_leg(args.config4_end_time) if args.leg == "n256"
-               else run_second_period_leg(limit=args.leg_time_limit))
+               else run_second_period_leg(limit=args.leg_time_limit, cap=args.attempt_cap))
         print(json.dumps({"leg": args.leg, **rec}), flush=True)
         return 0
     if args.gpus != 1:
@@ -744,6 +753,12 @@ def main():
         line["bicg_residual_1e-13"] = c3
         log("residual 1e-13 legs done")
 
+    if not args.plain_only and fits(40.0, "second_period_capped"):
+        # The steady state: period 2 costs ~5x period 1, and a few systems of the stiff corner
+        # reach the Mach-1 singularity there; they end at an attempt cap of 1e5.
+        log("second period leg (attempt cap 1e5)")
+        line["second_period_capped"] = run_second_period_leg(limit=60.0, cap=100000)
+        log("second period leg done")
     if args.pool_end_time > 0 and not args.plain_only and fits(60.0, "pool"):
         log("pool leg")
         line["pool"] = run_pool_leg(args.pool_end_time,
