@@ -13,6 +13,7 @@ import numpy as np
 from . import _native
 from . import collapse
 from . import workloads as wl
+from .problem_solver import LAYOUT_AUTO, LAYOUT_SYSTEM_PAIR
 from .problem_solver import ListOfSolverOptions as O
 from .problem_solver import ListOfVariables as V
 
@@ -27,36 +28,73 @@ def cost_proxy(ens):
     return np.asarray(ens.pa_bar, dtype=np.float64) / np.asarray(ens.f_khz, dtype=np.float64)
 
 
+def measured_cost(stats, n):
+    """Per-system cost measured by a previous solve: the FP64 FLOPs of its RK stages, BiCG
+    iterations and step attempts (per stage 4N^2 + 47N, per iteration 4N^2 + 21N, per attempt
+    140N) from the solver's counters ([NS][32], stats())."""
+    st = np.asarray(stats, dtype=np.float64)
+    return (st[:, 2] * (4 * n * n + 47 * n) + st[:, 3] * (4 * n * n + 21 * n) +
+            st[:, 10] * 140 * n)
+
+
+def hand_out_order(cost, group=1):
+    """Launch order of a part's systems (indices into it), costliest first: systems are taken in
+    index order in groups of `group` (the systems that share a wave; neighbouring grid points
+    cost about the same), and the whole groups are sorted by their costliest member; an
+    incomplete last group stays last, so that every wave keeps its group."""
+    cost = np.asarray(cost, dtype=np.float64)
+    n = cost.shape[0]
+    ng = n // group
+    gcost = cost[: ng * group].reshape(ng, group).max(axis=1)
+    gorder = np.argsort(-gcost, kind="stable")
+    idx = (gorder[:, None] * group + np.arange(group)[None, :]).reshape(-1)
+    return np.concatenate([idx, np.arange(ng * group, n)]).astype(np.int64)
+
+
 class LearnedBatchedSolver:
     """Learned + batched: the `heavy` systems with the largest cost_proxy run with the learned
-    BiCG start, the others with the plain start, as two solver objects (each on its own stream)
-    launched together, the heavy ones first. The plain kernel keeps its occupancy for the bulk
-    (3 waves per SIMD against the learned kernel's 2) and pays no network evaluation where the
-    start saves nothing; the learned start shortens the systems that bound the launch.
+    BiCG start (one system per wave, role pair), the others with the plain start on the
+    system-pair layout (two systems per wave: the right-hand side, reductions and controller run
+    once for both), as two solver objects (each on its own stream) launched together, the heavy
+    ones first. The bulk pays no network evaluation where the start saves nothing and runs at
+    the system-pair layout's throughput; the learned start shortens the systems that bound the
+    launch. With network=None the heavy systems take the plain start too (a plain batched split,
+    bit-identical to the plain solve system by system).
 
     Solve() is asynchronous like ProblemSolver.Solve(); synchronise() waits for both parts;
     results() returns state, accessories, event counts and counters in the ensemble's order.
     """
 
-    def __init__(self, ens, network, heavy=64, end_time=1.0, ndo=0, options=None):
+    def __init__(self, ens, network, heavy=64, end_time=1.0, ndo=0, options=None,
+                 light_layout=LAYOUT_SYSTEM_PAIR, cost=None):
+        """cost: per-system cost estimate that ranks the systems (default cost_proxy(ens); the
+        counters of a previous solve give measured_cost). Each part's persistent launch hands
+        its systems out costliest first, in pairs of neighbouring grid points on the
+        system-pair layout."""
         if ens.NS < 2:
             raise ValueError("a learned + batched split needs at least two systems")
-        order = np.argsort(-cost_proxy(ens), kind="stable")
+        cost = cost_proxy(ens) if cost is None else np.asarray(cost, dtype=np.float64)
+        if cost.shape != (ens.NS,):
+            raise ValueError("cost must give one value per system")
+        order = np.argsort(-cost, kind="stable")
         heavy = max(1, min(int(heavy), ens.NS - 1))
         self.heavy = np.sort(order[:heavy])
         self.light = np.sort(order[heavy:])
         self.NS, self.N = ens.NS, ens.N
         self.parts = []
-        for sids, net in ((self.heavy, network), (self.light, None)):
+        for sids, net, layout in ((self.heavy, network, LAYOUT_AUTO),
+                                  (self.light, None, light_layout)):
             sub = ens.subset(sids)
             s = collapse.make_solver(sub, NDO=ndo)
             collapse.fill(s, sub)
             s.host_view(V.TimeDomain)[sub.NS:] = end_time
+            s.SolverOption(O.KernelLayout, layout)
             if options:
                 collapse.apply_options(s, options)
             if net is not None:
                 s.set_warm_start_network(net)
                 s.SolverOption(O.WarmStart, 1)
+            s.set_launch_order(hand_out_order(cost[sids], 2 if layout == LAYOUT_SYSTEM_PAIR else 1))
             s.SynchroniseFromHostToDevice(V.All)
             self.parts.append((sids, s))
 

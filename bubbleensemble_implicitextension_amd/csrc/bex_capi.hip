@@ -73,6 +73,10 @@ struct bex_solver {
   double BiCGRhoTolerance = 1e-6, BiCGResidualTolerance = 1e-6;
   int BiCGMaxIterations = 19;
   int KernelLayout = BEX_LAYOUT_AUTO;
+  int PersistentLaunch = 1;  // 1: persistent waves over a device work queue (bex_solve)
+  unsigned long long* d_queue = nullptr;  // its counter
+  int* d_order = nullptr;                 // hand-out order of the systems (nullptr: index order)
+  int cu_count = 0;
   int PriorityStepThreshold = 3000;
   long long MaxStepAttempts = 0;  // 0: no cap
   int WarmStart = 0;                          // 1: learned BiCG initial guess
@@ -119,13 +123,22 @@ int bound(bex_solver* s, const char* fn, const char* what, long long v, long lon
   return BEX_OK;
 }
 
+// Clears a work-queue counter (a kernel rather than a memset, so that launch sequences can be
+// captured into graphs).
+__global__ void zero_queue_kernel(unsigned long long* c) { *c = 0ull; }
+
 using KernelFn = void (*)(BexKernelArgs, long long);
+// Persistent kernel: (args, order, systems, work items, queue counter).
+using QueueFn = void (*)(BexKernelArgs, const int*, long long, long long, unsigned long long*);
 
 struct KernelChoice {
-  KernelFn fn;
+  KernelFn fn;      // one workgroup per kSystemsPerBlock systems
+  QueueFn queue;    // persistent waves over a device work queue (same results)
   size_t lds;
   int systems_per_block;
   int block;
+  int items_per_block;
+  int systems_per_item;
 };
 
 // Dense output compiles into separate kernels (none, step points, Hermite), so launches without
@@ -138,10 +151,21 @@ KernelFn kernel_of(int dense) {
     default: return (KernelFn)bexk::solve_kernel<L, ALG, bexk::kDenseNone>;
   }
 }
+template <class L, int ALG>
+QueueFn queue_kernel_of(int dense) {
+  switch (dense) {
+    case bexk::kDenseSteps: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseSteps>;
+    case bexk::kDenseHermite: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseHermite>;
+    default: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseNone>;
+  }
+}
 template <class L>
 KernelChoice choice(int alg, int dense) {
-  const KernelFn fn = alg == BEX_RK4 ? kernel_of<L, bexk::kRK4>(dense) : kernel_of<L, bexk::kRKCK45>(dense);
-  return {fn, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock};
+  const bool rk4 = alg == BEX_RK4;
+  const KernelFn fn = rk4 ? kernel_of<L, bexk::kRK4>(dense) : kernel_of<L, bexk::kRKCK45>(dense);
+  const QueueFn q = rk4 ? queue_kernel_of<L, bexk::kRK4>(dense) : queue_kernel_of<L, bexk::kRKCK45>(dense);
+  return {fn, q, L::lds_bytes(), L::kSystemsPerBlock, L::kBlock, L::kItemsPerBlock,
+          L::kSystemsPerItem};
 }
 
 // Dense-output kernel variant of a solver's current settings.
@@ -149,9 +173,16 @@ int dense_mode(const bex_solver* s);
 
 // layout: BEX_LAYOUT_AUTO picks the role-pair layouts (32 units: one system per wave; 64 and
 // 128 units: two and eight waves per system, the matrix in registers) and the lane-per-unit
-// layouts otherwise; BEX_LAYOUT_GROUP forces lane-per-unit (the matrix in LDS). mode:
-// bexk::WarmStartMode; the learned guess and sample collection exist for 32 units only.
+// layouts otherwise; BEX_LAYOUT_GROUP forces lane-per-unit (the matrix in LDS);
+// BEX_LAYOUT_SYSTEM_PAIR takes two 32-unit systems per wave with both matrix rows in registers
+// (plain start only). mode: bexk::WarmStartMode; the learned guess and sample collection exist
+// for 32 units only.
 bool kernel_for(int ups, int alg, int layout, int mode, int dense, KernelChoice* k) {
+  if (layout == BEX_LAYOUT_SYSTEM_PAIR) {
+    if (ups != 32 || mode != bexk::kPlain) return false;
+    *k = choice<bexk::PairLayout32<false>>(alg, dense);
+    return true;
+  }
   if ((ups == 64 || ups == 128) && layout != BEX_LAYOUT_GROUP) {
     if (mode != bexk::kPlain) return false;
     if (ups == 64)
@@ -561,6 +592,8 @@ int bex_create(const bex_problem_desc* desc, int device, bex_solver** out) {
   ok = ok && hipMalloc((void**)&s->d_tol, sizeof(double) * ntol) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_dir, sizeof(int) * (d.NE > 0 ? d.NE : 1)) == hipSuccess;
   ok = ok && hipMalloc((void**)&s->d_stats, sizeof(long long) * (size_t)d.NS * BEX_NSTAT) == hipSuccess;
+  ok = ok && hipMalloc((void**)&s->d_queue, sizeof(unsigned long long)) == hipSuccess;
+  ok = ok && hipDeviceGetAttribute(&s->cu_count, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&s->h_stop, sizeof(int),
                            hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess;
   ok = ok && hipHostGetDevicePointer((void**)&s->d_stop, s->h_stop, 0) == hipSuccess;
@@ -619,6 +652,8 @@ int bex_destroy(bex_solver* s) {
   if (s->d_tol) hipFree(s->d_tol);
   if (s->d_dir) hipFree(s->d_dir);
   if (s->d_stats) hipFree(s->d_stats);
+  if (s->d_queue) hipFree(s->d_queue);
+  if (s->d_order) hipFree(s->d_order);
   if (s->h_timeline) hipHostFree(s->h_timeline);
   if (s->h_stop) hipHostFree(s->h_stop);
   if (s->d_DenseAux) hipFree(s->d_DenseAux);
@@ -665,6 +700,7 @@ int bex_solver_option(bex_solver* s, int option, double v) {
         return fail(s, BEX_E_ARGUMENT, "SolverOption: MaximumStepAttempts must be >= 0");
       s->MaxStepAttempts = (long long)v;
       break;
+    case BEX_PersistentLaunch: s->PersistentLaunch = (v != 0.0) ? 1 : 0; break;
     case BEX_KernelLayout: {
       KernelChoice kc;
       if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, bexk::kDenseNone, &kc))
@@ -1016,12 +1052,59 @@ int bex_solve(bex_solver* s) {
                 "Solve: interpolated dense output needs DenseOutputMinimumTimeStep > 0");
   int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
   if (nsys <= 0) return BEX_OK;
+  if (s->PersistentLaunch) {
+    // As many workgroups as fit on the device at once, each taking work items from the queue
+    // (the counter is cleared by a kernel, so the sequence can be captured into a graph).
+    // With a hand-out order every system is listed; without one, systems 0 .. nsys-1.
+    if (kc.lds > 65536)
+      BEX_HIP(s, hipFuncSetAttribute(reinterpret_cast<const void*>(kc.queue),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kc.lds));
+    const long long count = s->d_order ? (long long)s->d.NS : (long long)nsys;
+    const long long nitems = (count + kc.systems_per_item - 1) / kc.systems_per_item;
+    int per_cu = 0;
+    BEX_HIP(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &per_cu, reinterpret_cast<const void*>(kc.queue), kc.block, kc.lds));
+    if (per_cu < 1) return fail(s, BEX_E_UNSUPPORTED, "Solve: the kernel does not fit on a CU");
+    long long blocks = (long long)s->cu_count * per_cu;
+    const long long need = (nitems + kc.items_per_block - 1) / kc.items_per_block;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, s->stream, s->d_queue);
+    hipLaunchKernelGGL(kc.queue, dim3((unsigned)blocks), dim3(kc.block), kc.lds, s->stream, a,
+                       (const int*)s->d_order, count, nitems, s->d_queue);
+    BEX_HIP(s, hipGetLastError());
+    return BEX_OK;
+  }
   if (kc.lds > 65536)
     BEX_HIP(s, hipFuncSetAttribute(reinterpret_cast<const void*>(kc.fn),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kc.lds));
   dim3 grid((nsys + kc.systems_per_block - 1) / kc.systems_per_block), block(kc.block);
   hipLaunchKernelGGL(kc.fn, grid, block, kc.lds, s->stream, a, (long long)nsys);
   BEX_HIP(s, hipGetLastError());
+  return BEX_OK;
+}
+
+int bex_set_launch_order(bex_solver* s, const int* order, int n) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  if (!order || n == 0) {
+    if (s->d_order) {
+      BEX_HIP(s, hipStreamSynchronize(s->stream));
+      BEX_HIP(s, hipFree(s->d_order));
+      s->d_order = nullptr;
+    }
+    return BEX_OK;
+  }
+  if (n != s->d.NS)
+    return fail(s, BEX_E_ARGUMENT, "bex_set_launch_order: the order must list every system once");
+  std::vector<char> seen((size_t)n, 0);
+  for (int k = 0; k < n; k++) {
+    if (order[k] < 0 || order[k] >= n || seen[(size_t)order[k]])
+      return fail(s, BEX_E_ARGUMENT, "bex_set_launch_order: the order must list every system once");
+    seen[(size_t)order[k]] = 1;
+  }
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  if (!s->d_order) BEX_HIP(s, hipMalloc((void**)&s->d_order, sizeof(int) * (size_t)n));
+  BEX_HIP(s, hipMemcpy(s->d_order, order, sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
   return BEX_OK;
 }
 
@@ -1344,6 +1427,7 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<16, 4, true>>("group16 x4 waves")};
   static const std::vector<PoolVariant> v32 = {
       pool_variant<bexk::RolePairLayout32<bexk::kPlain, true>>("rolepair32"),
+      pool_variant<bexk::PairLayout32<true>>("pair32"),
       pool_variant<bexk::GroupLayout<32, 1, true>>("group32 x1 wave"),
       pool_variant<bexk::GroupLayout<32, 2, true>>("group32 x2 waves")};
   static const std::vector<PoolVariant> v64 = {
@@ -1369,8 +1453,6 @@ const std::vector<PoolVariant>& pool_variants(int n) {
     default: return none;
   }
 }
-
-__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ull; }
 
 }  // namespace
 
@@ -1431,7 +1513,7 @@ int launch_class(bex_pool* p, int c, hipStream_t st) {
   long long blocks = (long long)p->cu_count * per_cu;
   const long long need = (nitems + v.items_per_block - 1) / v.items_per_block;
   if (blocks > need) blocks = need;
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, k.d_counter);
+  hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, st, k.d_counter);
   hipLaunchKernelGGL(v.fn, dim3((unsigned)blocks), dim3(v.block), v.lds, st, a,
                      (const int*)k.d_order, (long long)k.NS, nitems, k.d_counter);
   BEX_PHIP(p, hipGetLastError());

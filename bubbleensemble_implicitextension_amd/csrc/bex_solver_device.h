@@ -1307,6 +1307,175 @@ struct RolePairLayout32 {
 };
 
 // =========================================================================================
+// PairLayout32: two N = 32 systems per wave, lane = unit. Lanes 0-31 serve one system, lanes
+// 32-63 the next, and every lane holds row i of M and row i of M^T in registers (128 VGPRs: the
+// shared coupling matrix, or with PS each system's own). A product M z runs as in the role
+// pair (two v_permlane16_swap and 32 v_fmac_f64 with a DPP row_newbcast operand, which never
+// cross a 32-lane half), so one pass computes M z for both systems and a second pass M^T w; the
+// halving trees, the divisions, the right-hand side, the controller and the event logic run once
+// for the two systems, where the role pair runs them once per system. The two systems step
+// independently: every system-level decision is taken per half and their branches diverge
+// through the exec mask, so a wave costs the slower of its two systems (systems at adjacent
+// grid points cost within about 1 % of each other). The BiCG arithmetic is group_coupling's,
+// i.e. the canonical order; the unit parameters live in LDS ([lane][NUP]).
+template <bool PS = false>
+struct PairLayout32 {
+  static constexpr int kUnits = 32;
+  static constexpr int kSystemsPerWave = 2;
+  static constexpr int kWaves = 1;
+  static constexpr int kBlock = kWave;
+  static constexpr int kSystemsPerBlock = 2;
+  static constexpr int kWavesPerEU = 2;  // 256 VGPRs: the two matrix rows take half of them
+  static constexpr int kMode = kPlain;
+  static constexpr bool kPerSystemM = PS;
+  static constexpr int kItemsPerBlock = 1;
+  static constexpr int kSystemsPerItem = 2;
+  static constexpr bool kUnitParamsInLds = true;
+  static constexpr int kNup = BexSystemDims::NUP > 0 ? BexSystemDims::NUP : 1;
+
+  static constexpr size_t lds_bytes() {
+    return sizeof(double) * kWave * kNup + 2 * sizeof(Sys) + 2 * sizeof(int) * BEX_NSTAT;
+  }
+
+  int lane, grp, sid_;
+  double mrow[32];   // row i of M
+  double mtrow[32];  // row i of M^T
+  Sys* sys;
+  int* stat;
+  double* upar_lds;
+
+  __device__ __forceinline__ double* unit_params() const { return upar_lds + lane * kNup; }
+
+  __device__ __forceinline__ void init(unsigned char* smem, const BexKernelArgs&) {
+    lane = threadIdx.x;
+    grp = lane >> 5;
+    upar_lds = reinterpret_cast<double*>(smem);
+    Sys* s = reinterpret_cast<Sys*>(smem + sizeof(double) * kWave * kNup);
+    sys = s + grp;
+    stat = reinterpret_cast<int*>(s + 2) + grp * BEX_NSTAT;
+  }
+
+  // One work item = two systems: first_system + 2 item + half, or the order array's entries
+  // 2 item and 2 item + 1. Loads the lane's rows of M and M^T and clears the half's counters.
+  __device__ __forceinline__ void bind(const BexKernelArgs& a, const int* order, long long item,
+                                       long long count) {
+    const long long pos = item * 2 + grp;
+    int s = -1;
+    if (pos < count) s = order ? order[pos] : a.first_system + (int)pos;
+    sid_ = s;
+    const int i = lane & 31;
+    if (i < BEX_NSTAT) stat[i] = 0;
+    if (s >= 0 && s < a.NS) {
+      if (PS) {
+        const double* m = a.CouplingMatrix + (size_t)s * 1024;
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+          mrow[j] = m[i + j * 32];
+          mtrow[j] = m[j + i * 32];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+          mrow[j] = a.CouplingMatrix[i + j * 32];
+          mtrow[j] = a.CouplingMatrixT[i + j * 32];
+        }
+      }
+    }
+    wave_sync();
+  }
+
+  __device__ __forceinline__ int sid() const { return sid_; }
+  __device__ __forceinline__ int uid() const { return lane & 31; }
+  __device__ __forceinline__ bool primal() const { return true; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+
+  __device__ __forceinline__ unsigned long long mask() const {
+    return grp ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+  }
+  // A half is either wholly active or wholly inactive, so the ballot over its 32 lanes is the
+  // system's.
+  __device__ __forceinline__ bool all(bool p) const { return (__ballot(p) & mask()) == mask(); }
+  __device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask()) != 0ull; }
+  // Halving-tree reductions of the half (Steppers.cuh:733-742) as butterflies: every lane of a
+  // half ends with its system's value.
+  __device__ __forceinline__ double sum(double v) const { return tree32(v, AddOp()); }
+  // Minimum over the system's units (positive, non-NaN operands: the controller's tol/err
+  // ratios and the event step candidates, as in the role pair).
+  __device__ __forceinline__ double min_keep(double v) const { return tree32(v, VminOp()); }
+
+  // sum_j row[j] z_j over the lane's half in the canonical order (four partial sums).
+  __device__ __forceinline__ static double row_product(double z, const double* row) {
+    double even, odd;
+    rows_of_half(z, &even, &odd);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    fmac_row16(acc, even, &row[0]);
+    fmac_row16(acc, odd, &row[16]);
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+
+  // max |r| < tol over the half, NaN ignored (the reference's NaN-ignoring max), from two
+  // compare masks as in the role pair.
+  __device__ __forceinline__ bool res_test(double res, double tol) const {
+    const unsigned long long numbers = __builtin_amdgcn_fcmp(res, res, 7);
+    const unsigned long long failing = __builtin_amdgcn_fcmp(bex::fabs_d(res), tol, 11) & numbers;
+    const unsigned f = grp ? (unsigned)(failing >> 32) : (unsigned)failing;
+    const unsigned n = grp ? (unsigned)(numbers >> 32) : (unsigned)numbers;
+    return f == 0u && n != 0u;
+  }
+
+  // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
+  // (Steppers.cuh:489-561), group_coupling's arithmetic with the loop rotated as in rp_bicg.
+  template <class Feat>
+  __device__ __forceinline__ BicgOut coupling(double* F, double x1, double cpt, double rD,
+                                              const BexKernelArgs& a, Feat&&, bool) const {
+    const double x1sq = x1 * x1;
+    const double xu = F[1];
+    const double mc = row_product(cpt, mrow);
+    const double mx = row_product(x1sq * xu, mrow);
+    const double b = fma(rD, mc, xu);
+    const double ax = fma(rD, mx, xu);
+    double r = -ax + b;
+    double rt = r;
+    double x = xu;
+    BicgOut out{0, false, true, 0.0};
+    double rho1 = sum(r * rt);
+    const int maxit = a.BiCGMaxIterations;
+    if (maxit >= 1) {
+      if (bex::fabs_d(rho1) < a.BiCGRhoTolerance) {
+        out.rho_exit = true;
+        out.maxit = false;
+      } else {
+        double p = r, pt = rt;
+        for (int iter = 1;; iter++) {
+          const double a1 = row_product(x1sq * p, mrow);
+          const double a2 = row_product(rD * pt, mtrow);
+          const double q = fma(rD, a1, p);
+          const double qt = fma(x1sq, a2, pt);
+          out.iterations++;
+          const double alpha = rho1 / sum(pt * q);
+          x = fma(alpha, p, x);
+          r = fma(-alpha, q, r);
+          rt = fma(-alpha, qt, rt);
+          const double rho2 = rho1;
+          const bool converged = res_test(r, a.BiCGResidualTolerance);
+          rho1 = sum(r * rt);
+          if (converged) {
+            out.maxit = false;
+            break;
+          }
+          if (iter >= maxit) break;
+          const double beta = rho1 / rho2;
+          p = fma(beta, p, r);
+          pt = fma(beta, pt, rt);
+        }
+      }
+    }
+    F[1] = x;
+    return out;
+  }
+};
+
+// =========================================================================================
 // ChainLayout128: one N = 128 system per workgroup of eight waves, the coupling matrix in
 // registers. Wave w serves the 32 units of block B = w / 2 as a role pair (primal half: M and
 // r, p, q, x; shadow half: M^T and r~, p~, q~) and runs two of the four FMA chains of every row
@@ -2371,9 +2540,13 @@ void solve_kernel(const BexKernelArgs a, long long count) {
 
 // Persistent variant: a fixed grid whose waves (or workgroups, for multi-wave systems) take
 // work items from a device queue until it is empty. order lists the systems in the order they
-// are handed out (longest expected run first); items are kSystemsPerItem consecutive entries.
-// *counter must be zero at launch; every wave exits once the counter passes nitems.
-template <class L, int ALG>
+// are handed out (longest expected run first; nullptr: first_system, first_system + 1, ...);
+// items are kSystemsPerItem consecutive entries. *counter must be zero at launch; every wave
+// exits once the counter passes nitems. A slot that frees up takes the next item at once,
+// where the hardware's in-order workgroup dispatch of solve_kernel leaves slots idle once the
+// systems' run times diverge. Pools run it without dense output, ProblemSolver::Solve() in
+// every dense mode.
+template <class L, int ALG, int DENSE = kDenseNone>
 __global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
 void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long count,
                  long long nitems, unsigned long long* counter) {
@@ -2390,7 +2563,7 @@ void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long
     lay.sync();
     if (item >= nitems) break;
     lay.bind(a, order, item, count);
-    integrate<L, ALG, kDenseNone>(lay, a);  // pools run without dense output
+    integrate<L, ALG, DENSE>(lay, a);
     lay.sync();
   }
 }

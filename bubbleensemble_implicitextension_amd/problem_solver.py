@@ -66,9 +66,10 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     WarmStart = 19
     DenseOutputInterpolation = 20
     MaximumStepAttempts = 21
+    PersistentLaunch = 22
 
 
-LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR, LAYOUT_SYSTEM_PAIR = 0, 1, 2, 3
 MLP_NPARAM = 4609  # warm-start network parameters (csrc/bex_mlp.h)
 SAMPLE_FLOATS = 10  # floats per training-sample row
 
@@ -251,6 +252,18 @@ class ProblemSolver:
     def checkpoint_restore(self):
         """Copies the last device snapshot back (asynchronous on the solver's stream)."""
         self._check(self._lib.bex_checkpoint_restore(self._h))
+
+    # -- persistent launch (no reference counterpart) ------------------------------------
+    def set_launch_order(self, order):
+        """Order in which Solve()'s persistent launch hands out the systems: every system
+        index once, longest expected run first; None returns to index order. Results do not
+        depend on it."""
+        if order is None:
+            self._check(self._lib.bex_set_launch_order(self._h, None, 0))
+            return
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        self._check(self._lib.bex_set_launch_order(
+            self._h, o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(o.size)))
 
     # -- counters ----------------------------------------------------------------------
     def stats(self):

@@ -69,7 +69,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   PriorityStepThreshold,
   WarmStart,
   DenseOutputInterpolation,
-  MaximumStepAttempts
+  MaximumStepAttempts,
+  PersistentLaunch
 };
 
 // --- Names of the enumerators (Interface.cuh:70-71, 2035-2106) ----------------------------
@@ -83,7 +84,7 @@ inline std::string SolverOptionsToString(ListOfSolverOptions Option) {
       "SharedGlobalVariables", "SharedCouplingMatrices", "BiCGRhoTolerance",
       "BiCGResidualTolerance", "BiCGMaxIterations",      "KernelLayout",
       "PriorityStepThreshold", "WarmStart",              "DenseOutputInterpolation",
-      "MaximumStepAttempts"};
+      "MaximumStepAttempts",   "PersistentLaunch"};
   const int i = static_cast<int>(Option) - static_cast<int>(InitialTimeStep);
   const int n = static_cast<int>(sizeof(names) / sizeof(names[0]));
   return (i >= 0 && i < n) ? names[i] : "Non-existent solver option!";
@@ -595,6 +596,14 @@ class ProblemSolver {
   // reference checkpoints through the host, Interface.cuh:1310-1328).
   void CheckpointSave() { Check(bex_checkpoint_save(handle_), "CheckpointSave"); }
   void CheckpointRestore() { Check(bex_checkpoint_restore(handle_), "CheckpointRestore"); }
+
+  // Hand-out order of the persistent launch (no reference counterpart): every system once,
+  // longest expected run first; an empty vector returns to index order.
+  void SetLaunchOrder(const std::vector<int>& order) {
+    Check(bex_set_launch_order(handle_, order.empty() ? nullptr : order.data(),
+                               static_cast<int>(order.size())),
+          "SetLaunchOrder");
+  }
 };
 
 // Uneven ensembles (no reference counterpart): size classes of systems with their own

@@ -98,7 +98,14 @@ enum bex_option {
    * default) runs every system to the end of its time domain as the reference does. Bounds a
    * launch whose systems reach the Keller-Miksis Mach-1 singularity (millions of forced steps
    * at the minimum step size). No reference counterpart. */
-  BEX_MaximumStepAttempts
+  BEX_MaximumStepAttempts,
+  /* 1 (the default): Solve() launches as many workgroups as fit on the device at once, and
+   * each takes its next systems from a device work queue when it finishes its current ones
+   * (hand-out order: bex_set_launch_order, else system index); 0: one workgroup per group of
+   * systems, dispatched by the hardware in index order, as the reference launches
+   * (Interface.cuh:1949-1956). The results are the same; the queue keeps every slot busy when
+   * the systems' run times differ. No reference counterpart. */
+  BEX_PersistentLaunch
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32, 64 and 128 units, lane groups otherwise */
@@ -106,6 +113,8 @@ enum bex_option {
 #define BEX_LAYOUT_ROLE_PAIR 2 /* 32 units: one system per wave, M / M^T rows in registers;
                                   64 and 128 units: two and eight waves per system, M / M^T in
                                   registers */
+#define BEX_LAYOUT_SYSTEM_PAIR 3 /* 32 units: two systems per wave (lane = unit), both rows of
+                                    M and M^T in registers; plain BiCG start only */
 
 /* The reference's template parameters (Interface.cuh:135). */
 typedef struct {
@@ -185,6 +194,11 @@ int bex_checkpoint_restore(bex_solver* s);
  * whose stiffest systems need millions of steps. */
 int bex_request_stop(bex_solver* s);
 int bex_clear_stop(bex_solver* s);
+
+/* Order in which the persistent launch hands out the systems (no reference counterpart):
+ * order[0..n-1] lists every system index once (n = NS), longest expected run first for the
+ * best balance; order = NULL or n = 0 returns to index order. Results do not depend on it. */
+int bex_set_launch_order(bex_solver* s, const int* order, int n);
 
 /* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);
  * copies device counters to `out` (synchronous). No reference counterpart. */

@@ -7,13 +7,15 @@ from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables 
 
 
 def run_gpu(ens, periods, algorithm=1, options=None, ndo=0, reset_after=None, end_time=1.0,
-            layout=0):
+            layout=0, setup=None):
     s = collapse.make_solver(ens, algorithm=algorithm, NDO=ndo)
     collapse.fill(s, ens)
     s.SolverOption(O.KernelLayout, layout)
     s.host_view(V.TimeDomain)[ens.NS:] = end_time
     if options:
         collapse.apply_options(s, options)
+    if setup:
+        setup(s)
     s.SynchroniseFromHostToDevice(V.All)
     for k in range(periods):
         if reset_after is not None and k == reset_after:

@@ -136,6 +136,12 @@ def test_error_conventions():
         s.GetHost(0, V.ActualState, 0)  # a unit-scope variable through the system overload
     s.SetHost(1, 2, V.ActualState, 1, 0.25)
     assert s.GetHost(1, 2, V.ActualState, 1) == 0.25
+    # The persistent launch's hand-out order must list every system exactly once.
+    for bad in ([0, 1, 2], [0, 1, 2, 2], [0, 1, 2, 4], [-1, 0, 1, 2]):
+        with pytest.raises(BexError, match="every system once"):
+            s.set_launch_order(bad)
+    s.set_launch_order([3, 2, 1, 0])
+    s.set_launch_order(None)
     s.close()
 
 

@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from bubbleensemble_implicitextension_amd import workloads as wl
-from bubbleensemble_implicitextension_amd.problem_solver import LAYOUT_GROUP, LAYOUT_ROLE_PAIR
+from bubbleensemble_implicitextension_amd.problem_solver import (LAYOUT_GROUP, LAYOUT_ROLE_PAIR,
+                                                                  LAYOUT_SYSTEM_PAIR)
 from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
 from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
@@ -18,7 +19,7 @@ from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
 pytestmark = pytest.mark.gpu
 
 CASES = [(8, LAYOUT_GROUP), (16, LAYOUT_GROUP), (32, LAYOUT_GROUP), (32, LAYOUT_ROLE_PAIR),
-         (64, LAYOUT_GROUP), (64, LAYOUT_ROLE_PAIR)]
+         (32, LAYOUT_SYSTEM_PAIR), (64, LAYOUT_GROUP), (64, LAYOUT_ROLE_PAIR)]
 
 
 @pytest.mark.parametrize("N,layout", CASES)
@@ -49,7 +50,8 @@ def test_wide_layouts_bitexact(oracle, N, end, layout):
 
 
 @pytest.mark.parametrize("N,layout", [(32, LAYOUT_GROUP), (32, LAYOUT_ROLE_PAIR),
-                                      (64, LAYOUT_ROLE_PAIR), (128, LAYOUT_ROLE_PAIR)])
+                                      (32, LAYOUT_SYSTEM_PAIR), (64, LAYOUT_ROLE_PAIR),
+                                      (128, LAYOUT_ROLE_PAIR)])
 def test_rk4_bitexact(oracle, N, layout):
     ens = wl.small_ensemble(n=N, ns=4 if N == 32 else 2, seed=5, pa=(0.3, 0.5), f=(150.0, 200.0))
     opts = {"InitialTimeStep": 2e-4}
@@ -66,7 +68,7 @@ def config0_subset_sids():
     return [s for s in range(64) if s != 7]
 
 
-@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR])
+@pytest.mark.parametrize("layout", [LAYOUT_GROUP, LAYOUT_ROLE_PAIR, LAYOUT_SYSTEM_PAIR])
 def test_config0_two_periods_with_reset(oracle, layout):
     ens = wl.config0().subset(config0_subset_sids())
     print(f"config0 subset NS={ens.NS} layout={layout}", flush=True)
@@ -159,6 +161,21 @@ def test_singular_corner_short_window(oracle):
     compare_solver_to_oracle(s, p)
 
 
+def test_system_pair_with_unequal_systems(oracle):
+    """Two systems of very different cost in one wave of the system-pair layout (the Mach-1
+    corner, 4.5x the steps, and a mild system, each way round, plus a third system alone in the
+    last wave), over a whole period: each half steps, rejects, records events and ends on its
+    own, bit for bit like the oracle."""
+    ens = wl.config0().subset([7, 0, 0, 7, 9])
+    print("case: system pair, corner + mild, tau in [0, 1]", flush=True)
+    s = run_gpu(ens, periods=1, layout=LAYOUT_SYSTEM_PAIR)
+    p = run_oracle(oracle, ens, periods=1)
+    compare_solver_to_oracle(s, p)
+    st = s.stats()
+    assert st[0, 0] > 4 * st[1, 0]  # the corner system takes many more steps
+    s.close()
+
+
 def test_config0_within_config1_batch(oracle):
     """A system's result does not depend on the batch it runs in: config 0's systems solved
     inside the 16384-system config 1 launch (over the first fifth of a period) equal the
@@ -183,13 +200,25 @@ def test_config1_full_size_layouts_and_reruns_agree():
     """Size-independent properties at the full 16384-system size: the role-pair kernel (rows of
     M in registers, DPP broadcasts, one system per wave) and the lane-group kernel (M in LDS, two
     systems per wave) agree bit for bit in state, accessories, event counts and counters over a
-    fifth of a period, and a second role-pair launch repeats the first bit for bit."""
+    fifth of a period, and so does the system-pair kernel (two systems per wave, rows of M and
+    M^T in registers); a second role-pair launch repeats the first bit for bit, and so do the
+    hardware-dispatched launch (one workgroup per system) and a persistent launch that hands
+    the systems out in reverse order."""
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+
     big = wl.config1()
-    print("case: config 1, 16384 systems x N=32, tau in [0, 0.2]: role pair twice, lane group "
-          "once", flush=True)
+    print("case: config 1, 16384 systems x N=32, tau in [0, 0.2]: role pair (persistent "
+          "twice, dispatched, reverse order), lane group, system pair", flush=True)
     runs = []
-    for layout in (LAYOUT_ROLE_PAIR, LAYOUT_ROLE_PAIR, LAYOUT_GROUP):
-        s = run_gpu(big, periods=1, end_time=0.2, layout=layout)
+    for layout, how in ((LAYOUT_ROLE_PAIR, None), (LAYOUT_ROLE_PAIR, None),
+                        (LAYOUT_ROLE_PAIR, "dispatched"), (LAYOUT_ROLE_PAIR, "reverse"),
+                        (LAYOUT_GROUP, None), (LAYOUT_SYSTEM_PAIR, None)):
+        def setup(s):
+            if how == "dispatched":
+                s.SolverOption(O.PersistentLaunch, 0)
+            elif how == "reverse":
+                s.set_launch_order(np.arange(big.NS)[::-1])
+        s = run_gpu(big, periods=1, end_time=0.2, layout=layout, setup=setup)
         runs.append([s.state().copy(), s.accessories().copy(), s.event_counts().copy(),
                      s.stats().copy()])
         s.close()
@@ -203,6 +232,7 @@ def test_config1_full_size_layouts_and_reruns_agree():
 
 @pytest.mark.parametrize("N,layout,periods,end", [(32, LAYOUT_GROUP, 2, 1.0),
                                                    (32, LAYOUT_ROLE_PAIR, 2, 1.0),
+                                                   (32, LAYOUT_SYSTEM_PAIR, 2, 1.0),
                                                    (64, LAYOUT_ROLE_PAIR, 2, 0.3),
                                                    (128, LAYOUT_ROLE_PAIR, 2, 0.1)])
 @pytest.mark.parametrize("interp", [0, 1])
