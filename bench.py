@@ -41,7 +41,7 @@ sys.path.insert(0, REPO)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak, dense (vector and matrix pipes alike)
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E peak bandwidth
 DENSE_POINTS = 64        # dense-output points per period of the headline workload
-KERNEL_NAME = "bexk::solve_kernel<RolePairLayout32<kPlain>, RKCK45, dense>"
+KERNEL_NAME = "bexk::queue_kernel<RolePairLayout32<kPlain>, RKCK45, dense>"
 
 
 def elapsed():
@@ -150,7 +150,7 @@ def measured_traffic():
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
     of this bench command (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's
     reading for a plain device copy of known size). None when the summary is absent."""
-    path = os.path.join(REPO, "profiles", "solve_kernel_n32_16384sys_dense64_pmc.md")
+    path = os.path.join(REPO, "profiles", "queue_kernel_rolepair32_16384sys_dense64_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -201,12 +201,17 @@ def make_headline_solver(ens, end_time, network=None, options=None, dense=True):
 
 
 def time_steps(target, steps, warmup, timer=None):
-    """Warmup, then `steps` timed (restore, Solve) pairs on `target` (a solver or a
-    learned+batched pair). Returns wall seconds and, with a timer solver, the HIP-event time of
-    the timed launches on its stream minus the same number of restores timed alone."""
+    """Warmup, then `steps` timed (restore, Solve) pairs on `target` (a solver or a split
+    solver). Returns wall seconds and, with a timer solver, the HIP-event time of the timed
+    launches on its stream minus the same number of restores timed alone. A split solver's
+    parts run on streams of their own; every step ends with all of them done (a period of the
+    whole ensemble), so no part of one step overlaps the next."""
+    split = hasattr(target, "parts")
     for _ in range(warmup):
         target.checkpoint_restore()
         target.Solve()
+        if split:
+            target.synchronise()
     target.synchronise() if hasattr(target, "synchronise") else target.SynchroniseDevice()
     restore_ms = 0.0
     if timer is not None:
@@ -222,6 +227,8 @@ def time_steps(target, steps, warmup, timer=None):
     for _ in range(steps):
         target.checkpoint_restore()
         target.Solve()
+        if split:
+            target.synchronise()
     event_ms = timer.timer_stop() if timer is not None else None
     target.synchronise() if hasattr(target, "synchronise") else target.SynchroniseDevice()
     wall = time.perf_counter() - t0
@@ -335,6 +342,51 @@ def wait_with_progress(pool, solvers, label, every=20.0, limit=None):
     for s in solvers:
         s.clear_stop()
     return finished_at_stop
+
+
+def run_split(ens, network, heavy, medium, steps, warmup, end_time, cost):
+    """Times `steps` from-rest periods of the three-part split (batched.LearnedBatchedSolver)
+    with dense output; returns wall seconds, the results and the parts."""
+    from bubbleensemble_implicitextension_amd import batched
+
+    lb = batched.LearnedBatchedSolver(ens, network, heavy=heavy, medium=medium,
+                                      end_time=end_time, ndo=DENSE_POINTS,
+                                      options=dense_options(end_time), cost=cost)
+    lb.checkpoint_save()
+    wall, _, _ = time_steps(lb, steps, warmup)
+    res = lb.results(dense=True)
+    parts = {"heavy": lb.heavy, "medium": lb.medium, "light": lb.light}
+    lb.close()
+    return wall, res, parts
+
+
+def split_text(parts, learned):
+    start = "learned" if learned else "plain"
+    return (f"{len(parts['heavy'])} systems with the largest forcing amplitude / frequency on the "
+            f"{start} start (one per wave), the next {len(parts['medium'])} by the cost measured "
+            "in the plain leg's solve on the plain start (one per wave), the other "
+            f"{len(parts['light'])} on the plain start two per wave (system-pair layout); three "
+            "solver objects launched together, each step ends when all three are done")
+
+
+def run_split_leg(steps, warmup, heavy, medium):
+    """The learned+batched split alone (profiling): one untimed plain solve measures the
+    systems' costs, then `steps` timed periods of the split."""
+    from bubbleensemble_implicitextension_amd import batched
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    ens = wl.config1()
+    s = make_headline_solver(ens, 1.0)
+    s.checkpoint_restore()
+    s.Solve()
+    s.SynchroniseDevice()
+    cost = batched.measured_cost(s.stats(), ens.N)
+    s.close()
+    net = np.load(batched.DEFAULT_NETWORK)
+    wall, res, parts = run_split(ens, net, heavy, medium, steps, warmup, 1.0, cost)
+    return {"value": ens.NS * steps / wall, "unit": "system-periods/s",
+            "ms_per_step": wall * 1e3 / steps, "split": split_text(parts, True),
+            "bicg_iterations_per_rhs": iters_per_rhs(res["stats"])}
 
 
 def run_pool_leg(end_time, probe_fraction=0.02, sequential=True, limit=None, cap=0):
@@ -547,6 +599,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned legs")
     ap.add_argument("--heavy", type=int, default=64,
                     help="systems on the learned start in the learned+batched leg")
+    ap.add_argument("--medium", type=int, default=192,
+                    help="next costliest systems (measured) on the plain start, one per wave, "
+                         "in the learned+batched leg")
     ap.add_argument("--pool-end-time", type=float, default=0.1,
                     help="period fraction of the config-2 pool leg (0 skips it); a full period "
                          "is dominated by the few systems that reach the Mach-1 singularity "
@@ -564,7 +619,7 @@ def main():
                     help="headline leg only (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration) and exit")
-    ap.add_argument("--leg", choices=("pool", "n256", "period2"),
+    ap.add_argument("--leg", choices=("pool", "n256", "period2", "split"),
                     help="run only this leg (with --pool-end-time / --config4-end-time) and "
                          "print its record instead of the benchmark line")
     args = ap.parse_args()
@@ -576,6 +631,8 @@ def main():
                             cap=args.attempt_cap)
                if args.leg == "pool"
                else run_config4_leg(args.config4_end_time) if args.leg == "n256"
+               else run_split_leg(args.steps, args.warmup, args.heavy, args.medium)
+               if args.leg == "split"
                else run_second_period_leg(limit=args.leg_time_limit, cap=args.attempt_cap))
         print(json.dumps({"leg": args.leg, **rec}), flush=True)
         return 0
@@ -696,25 +753,25 @@ def main():
         }
         del learned
         log("learned leg done")
+    # Learned + batched: three parts launched together (batched.LearnedBatchedSolver). The
+    # medium part is ranked by the cost each system measured in the plain leg's solve of this
+    # period (its counters); in a run of consecutive periods, the previous period's counters.
+    measured = batched.measured_cost(stats, N)
+
+    def split_leg(network):
+        return run_split(ens, network, args.heavy, args.medium, K, W, args.end_time, measured)
+
     if have_net and fits(plain_s, "learned_batched"):
         log("learned+batched leg")
-        lb = batched.LearnedBatchedSolver(ens, net, heavy=args.heavy, end_time=args.end_time,
-                                             ndo=DENSE_POINTS,
-                                             options=dense_options(args.end_time))
-        lb.checkpoint_save()
-        lwall, _, _ = time_steps(lb, K, W)
-        lbr = lb.results(dense=True)
-        heavy = lb.heavy
-        lb.close()
+        lwall, lbr, parts = split_leg(net)
+        heavy = parts["heavy"]
         lbs = lbr["stats"]
         lbv = ens.NS * K / lwall
         line["learned_batched"] = {
             "value": lbv, "unit": "system-periods/s",
             "ms_per_step": lwall * 1e3 / K,
             "speedup_vs_plain": lbv / value,
-            "split": f"{len(heavy)} systems with the largest forcing amplitude / frequency on "
-                     "the learned start, the others on the plain start, two solver objects "
-                     "launched together",
+            "split": split_text(parts, True),
             "learned_systems": [int(k) for k in heavy[:8]] + (["..."] if len(heavy) > 8 else []),
             "bicg_iterations_per_rhs": iters_per_rhs(lbs),
             "bicg_iterations_per_rhs_learned_systems": iters_per_rhs(lbs, heavy),
@@ -722,6 +779,20 @@ def main():
             "error_vs_plain": error_vs(plain, lbr),
         }
         log("learned+batched leg done")
+    if not args.plain_only and fits(plain_s, "plain_batched"):
+        # The same split with the plain start everywhere: bit-identical to the plain leg, and
+        # bound by the same stiffest system, which only the learned start shortens.
+        log("plain batched leg")
+        pwall, pbr, parts = split_leg(None)
+        same = all(np.array_equal(pbr[k].view(np.int64) if pbr[k].dtype.kind == "f" else pbr[k],
+                                  plain[k].view(np.int64) if plain[k].dtype.kind == "f" else plain[k])
+                   for k in ("state", "acc", "events"))
+        pbv = ens.NS * K / pwall
+        line["plain_batched"] = {
+            "value": pbv, "unit": "system-periods/s", "ms_per_step": pwall * 1e3 / K,
+            "speedup_vs_plain": pbv / value, "split": split_text(parts, False),
+            "bit_identical_to_plain": bool(same)}
+        log("plain batched leg done")
 
     # Config 3: the same shapes with BiCG run to an absolute residual of 1e-13.
     k3 = min(K, 3)

@@ -51,44 +51,66 @@ def hand_out_order(cost, group=1):
     return np.concatenate([idx, np.arange(ng * group, n)]).astype(np.int64)
 
 
-class LearnedBatchedSolver:
-    """Learned + batched: the `heavy` systems with the largest cost_proxy run with the learned
-    BiCG start (one system per wave, role pair), the others with the plain start on the
-    system-pair layout (two systems per wave: the right-hand side, reductions and controller run
-    once for both), as two solver objects (each on its own stream) launched together, the heavy
-    ones first. The bulk pays no network evaluation where the start saves nothing and runs at
-    the system-pair layout's throughput; the learned start shortens the systems that bound the
-    launch. With network=None the heavy systems take the plain start too (a plain batched split,
-    bit-identical to the plain solve system by system).
+def split_systems(ens, heavy, medium, cost):
+    """(heavy, medium, light) system indices, each sorted: the `heavy` largest cost_proxy, then
+    the `medium` largest `cost` among the others, then the rest (at least one system)."""
+    if ens.NS < 2:
+        raise ValueError("a learned + batched split needs at least two systems")
+    cost = np.asarray(cost, dtype=np.float64)
+    if cost.shape != (ens.NS,):
+        raise ValueError("cost must give one value per system")
+    heavy = max(1, min(int(heavy), ens.NS - 1))
+    medium = max(0, min(int(medium), ens.NS - 1 - heavy))
+    by_proxy = np.argsort(-cost_proxy(ens), kind="stable")
+    rest = np.sort(by_proxy[heavy:])
+    rest = rest[np.argsort(-cost[rest], kind="stable")]
+    return np.sort(by_proxy[:heavy]), np.sort(rest[:medium]), np.sort(rest[medium:])
 
-    Solve() is asynchronous like ProblemSolver.Solve(); synchronise() waits for both parts;
+
+class LearnedBatchedSolver:
+    """Learned + batched: the ensemble split into parts that run as separate solver objects
+    (each on its own stream), launched together, costliest part first:
+
+    * heavy: the `heavy` systems with the largest cost_proxy (the stiff corner of the forcing
+      grid, near the Keller-Miksis Mach-1 singularity) on the learned BiCG start, one system per
+      wave (role pair). There the learned start cuts the BiCG iterations and shortens the
+      systems that bound the launch; elsewhere it saves none and costs its network evaluation.
+    * medium: the next `medium` systems by `cost` on the plain start, one system per wave (role
+      pair: the shortest time per system).
+    * light: the rest on the plain start on the system-pair layout (two systems per wave: the
+      right-hand side, reductions and controller run once for both), the highest throughput.
+
+    cost ranks the systems for the medium part (default cost_proxy(ens); the counters of a
+    previous solve give measured_cost). Each part's persistent launch hands its systems out
+    costliest first; the heavy and medium waves issue at the top priority. With network=None
+    the heavy systems take the plain start too (a plain batched split, bit-identical to the
+    plain solve system by system).
+
+    Solve() is asynchronous like ProblemSolver.Solve(); synchronise() waits for every part;
     results() returns state, accessories, event counts and counters in the ensemble's order.
     """
 
     def __init__(self, ens, network, heavy=64, end_time=1.0, ndo=0, options=None,
-                 light_layout=LAYOUT_SYSTEM_PAIR, cost=None):
-        """cost: per-system cost estimate that ranks the systems (default cost_proxy(ens); the
-        counters of a previous solve give measured_cost). Each part's persistent launch hands
-        its systems out costliest first, in pairs of neighbouring grid points on the
-        system-pair layout."""
-        if ens.NS < 2:
-            raise ValueError("a learned + batched split needs at least two systems")
+                 light_layout=LAYOUT_SYSTEM_PAIR, cost=None, medium=0):
         cost = cost_proxy(ens) if cost is None else np.asarray(cost, dtype=np.float64)
-        if cost.shape != (ens.NS,):
-            raise ValueError("cost must give one value per system")
-        order = np.argsort(-cost, kind="stable")
-        heavy = max(1, min(int(heavy), ens.NS - 1))
-        self.heavy = np.sort(order[:heavy])
-        self.light = np.sort(order[heavy:])
+        self.heavy, self.medium, self.light = split_systems(ens, heavy, medium, cost)
         self.NS, self.N = ens.NS, ens.N
         self.parts = []
-        for sids, net, layout in ((self.heavy, network, LAYOUT_AUTO),
-                                  (self.light, None, light_layout)):
+        plan = [(self.heavy, network, LAYOUT_AUTO, True)]
+        if self.medium.size > 0:
+            plan.append((self.medium, None, LAYOUT_AUTO, True))
+        plan.append((self.light, None, light_layout, False))
+        for sids, net, layout, urgent in plan:
             sub = ens.subset(sids)
             s = collapse.make_solver(sub, NDO=ndo)
             collapse.fill(s, sub)
             s.host_view(V.TimeDomain)[sub.NS:] = end_time
             s.SolverOption(O.KernelLayout, layout)
+            if urgent:
+                # The heavy and medium systems bound the launch: their waves take the top issue
+                # priority from the first accepted step on (the light part keeps the default
+                # step threshold).
+                s.SolverOption(O.PriorityStepThreshold, 1)
             if options:
                 collapse.apply_options(s, options)
             if net is not None:

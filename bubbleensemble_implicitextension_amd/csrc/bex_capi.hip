@@ -154,9 +154,9 @@ KernelFn kernel_of(int dense) {
 template <class L, int ALG>
 QueueFn queue_kernel_of(int dense) {
   switch (dense) {
-    case bexk::kDenseSteps: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseSteps>;
-    case bexk::kDenseHermite: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseHermite>;
-    default: return (QueueFn)bexk::pool_kernel<L, ALG, bexk::kDenseNone>;
+    case bexk::kDenseSteps: return (QueueFn)bexk::queue_kernel<L, ALG, bexk::kDenseSteps>;
+    case bexk::kDenseHermite: return (QueueFn)bexk::queue_kernel<L, ALG, bexk::kDenseHermite>;
+    default: return (QueueFn)bexk::queue_kernel<L, ALG, bexk::kDenseNone>;
   }
 }
 template <class L>
@@ -1412,7 +1412,7 @@ struct PoolVariant {
 
 template <class L>
 PoolVariant pool_variant(const char* name) {
-  return {name, (PoolFn)bexk::pool_kernel<L, bexk::kRKCK45>, L::lds_bytes(), L::kBlock,
+  return {name, (PoolFn)bexk::queue_kernel<L, bexk::kRKCK45>, L::lds_bytes(), L::kBlock,
           L::kItemsPerBlock, L::kSystemsPerItem};
 }
 

@@ -2548,7 +2548,7 @@ void solve_kernel(const BexKernelArgs a, long long count) {
 // every dense mode.
 template <class L, int ALG, int DENSE = kDenseNone>
 __global__ __launch_bounds__(L::kBlock) __attribute__((amdgpu_waves_per_eu(L::kWavesPerEU)))
-void pool_kernel(const BexKernelArgs a, const int* __restrict__ order, long long count,
+void queue_kernel(const BexKernelArgs a, const int* __restrict__ order, long long count,
                  long long nitems, unsigned long long* counter) {
   extern __shared__ __align__(16) unsigned char smem[];
   L lay;

@@ -286,3 +286,25 @@ def test_dense_output_streaming_per_period(oracle):
         assert_bitwise(f"period {k} states", x, rs[41 * k: 41 * (k + 1)])
     s.close()
     ref.close()
+
+
+def test_batched_split_bit_identical_to_plain(oracle):
+    """The three-part split of batched.LearnedBatchedSolver with the plain start everywhere
+    (heavy and medium systems one per wave, the rest two per wave on the system-pair layout,
+    three solver objects on their own streams, costliest first) reproduces the oracle system by
+    system, over a whole period with dense output off."""
+    from bubbleensemble_implicitextension_amd import batched
+
+    ens = wl.config0().subset(config0_subset_sids())
+    cost = np.arange(ens.NS, dtype=np.float64)[::-1] % 7  # any ranking gives the same results
+    print(f"case: plain split of {ens.NS} systems, heavy 4, medium 9", flush=True)
+    lb = batched.LearnedBatchedSolver(ens, None, heavy=4, medium=9, cost=cost)
+    assert len(lb.parts) == 3
+    lb.Solve()
+    lb.synchronise()
+    got = lb.results()
+    lb.close()
+    p = run_oracle(oracle, ens, periods=1)
+    from gpu_helpers import assert_bitwise
+    assert_bitwise("state", got["state"], np.moveaxis(p.ActualState.reshape(2, ens.NS, ens.N), 0, -1))
+    assert_bitwise("stats", got["stats"], p.stats_table())
