@@ -82,6 +82,14 @@ def flops_fm1(stats, n):
     return stages * (4 * n * n + 47 * n) + iters * (4 * n * n + 21 * n) + attempts * 140 * n
 
 
+def fp64_rate(stats, n, wall):
+    """FM-1 FLOPs of the counted launches over their wall time, against the FP64 peak."""
+    achieved = flops_fm1(stats, n) / wall / 1e12
+    return {"achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "note": "FM-1 FLOPs of all parts over the wall time of the timed steps"}
+
+
 def iters_per_rhs(stats, rows=None):
     st = stats if rows is None else stats[rows]
     return float(st[:, 3].sum()) / max(1.0, float(st[:, 2].sum()))
@@ -776,6 +784,7 @@ def main():
             "bicg_iterations_per_rhs": iters_per_rhs(lbs),
             "bicg_iterations_per_rhs_learned_systems": iters_per_rhs(lbs, heavy),
             "bicg_iterations_per_rhs_learned_systems_plain_start": iters_per_rhs(stats, heavy),
+            "fp64": fp64_rate(lbs, N, lwall),
             "error_vs_plain": error_vs(plain, lbr),
         }
         log("learned+batched leg done")
@@ -791,6 +800,7 @@ def main():
         line["plain_batched"] = {
             "value": pbv, "unit": "system-periods/s", "ms_per_step": pwall * 1e3 / K,
             "speedup_vs_plain": pbv / value, "split": split_text(parts, False),
+            "fp64": fp64_rate(pbr["stats"], N, pwall),
             "bit_identical_to_plain": bool(same)}
         log("plain batched leg done")
 

@@ -308,3 +308,24 @@ def test_batched_split_bit_identical_to_plain(oracle):
     from gpu_helpers import assert_bitwise
     assert_bitwise("state", got["state"], np.moveaxis(p.ActualState.reshape(2, ens.NS, ens.N), 0, -1))
     assert_bitwise("stats", got["stats"], p.stats_table())
+
+
+@pytest.mark.parametrize("layout,persistent", [(LAYOUT_ROLE_PAIR, 1), (LAYOUT_ROLE_PAIR, 0),
+                                               (LAYOUT_SYSTEM_PAIR, 1), (LAYOUT_SYSTEM_PAIR, 0),
+                                               (LAYOUT_GROUP, 1)])
+def test_active_systems_subset(oracle, layout, persistent):
+    """ActiveSystems = 5 of 9 (Solver.cuh:82,132-138): the first five systems are integrated,
+    the others keep their state and counters untouched, in every launch mode; with the system
+    pair, system 4 shares its wave with the inactive system 5."""
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+
+    ens = wl.small_ensemble(n=32, ns=9, seed=21)
+    opts = {"ActiveSystems": 5}
+    print(f"case: ActiveSystems 5 of 9, layout {layout}, persistent {persistent}", flush=True)
+    s = run_gpu(ens, periods=1, end_time=0.3, layout=layout, options=opts,
+                setup=lambda s: s.SolverOption(O.PersistentLaunch, persistent))
+    p = run_oracle(oracle, ens, periods=1, end_time=0.3, options=opts)
+    compare_solver_to_oracle(s, p)
+    st = s.stats()
+    assert st[:5, 0].min() > 0 and st[5:].sum() == 0
+    s.close()
