@@ -197,7 +197,9 @@ int bex_clear_stop(bex_solver* s);
 
 /* Order in which the persistent launch hands out the systems (no reference counterpart):
  * order[0..n-1] lists every system index once (n = NS), longest expected run first for the
- * best balance; order = NULL or n = 0 returns to index order. Results do not depend on it. */
+ * best balance; order = NULL or n = 0 returns to index order. On layouts with several systems
+ * per wave, consecutive entries share a wave (for the system pair: entries 2k and 2k + 1), so
+ * systems of similar cost should be neighbours. Results do not depend on it. */
 int bex_set_launch_order(bex_solver* s, const int* order, int n);
 
 /* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);

@@ -306,7 +306,10 @@ def test_batched_split_bit_identical_to_plain(oracle):
     lb.close()
     p = run_oracle(oracle, ens, periods=1)
     from gpu_helpers import assert_bitwise
-    assert_bitwise("state", got["state"], np.moveaxis(p.ActualState.reshape(2, ens.NS, ens.N), 0, -1))
+    NS, N = ens.NS, ens.N
+    assert_bitwise("state", got["state"], np.moveaxis(p.ActualState.reshape(2, NS, N), 0, -1))
+    assert_bitwise("acc", got["acc"], np.moveaxis(p.UnitAccessories.reshape(-1, NS, N), 0, -1))
+    np.testing.assert_array_equal(got["events"], p.event_counts())
     assert_bitwise("stats", got["stats"], p.stats_table())
 
 
