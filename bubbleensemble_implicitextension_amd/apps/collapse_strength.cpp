@@ -7,8 +7,14 @@
 // Build (done by bubbleensemble_implicitextension_amd/build.py):
 //   g++ -O2 -std=c++17 -Iinclude collapse_strength.cpp -L<package dir> -lbex -o collapse_strength
 // Run:
-//   collapse_strength <radii file> <coupling file> <output file> [PAMIN PAMAX FRMIN FRMAX]
+//   collapse_strength <radii file> <coupling file> <output file> [PAMIN PAMAX FRMIN FRMAX
+//                     [HEAVY MEDIUM]]
+// With HEAVY and MEDIUM the same scan runs on the split solver (SplitProblemSolver: the HEAVY
+// systems with the largest amplitude / frequency and the next MEDIUM one per wave, the others
+// two per wave, all on the plain BiCG start); its result file equals the plain one.
+#include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -56,6 +62,8 @@ const int NDO = 0;
 
 using Solver = ProblemSolver<NS, UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP, NiGP, NUA, NiUA,
                              NSA, NiSA, NE, NDO, RKCK45, double>;
+using Split = SplitProblemSolver<NS, UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP, NiGP, NUA,
+                                 NiUA, NSA, NiSA, NE, NDO, RKCK45, double>;
 
 static std::vector<double> grid(double lo, double hi, int n, bool logarithmic) {
   std::vector<double> v(n, lo);
@@ -70,7 +78,8 @@ static std::vector<double> grid(double lo, double hi, int n, bool logarithmic) {
 }
 
 // Per-unit Keller-Miksis constants C0..C20 (CollapseStrength.cu:321-357).
-static void fill_system(Solver& s, int sid, double pa_bar, double f_khz,
+template <class S>
+static void fill_system(S& s, int sid, double pa_bar, double f_khz,
                         const std::vector<double>& radii_um) {
   const double PI = 3.14159265358979323846;
   const double Pv = 3.166775638952003e+03, Rho = 9.970639504998557e+02;
@@ -112,10 +121,16 @@ static void fill_system(Solver& s, int sid, double pa_bar, double f_khz,
   }
 }
 
+template <class S>
+static int run(S& s, const std::vector<double>& radii, const std::vector<double>& M,
+               const std::vector<double>& amp, const std::vector<double>& freq,
+               const char* output);
+
 int main(int argc, char** argv) {
   if (argc < 4) {
     std::cerr << "usage: " << argv[0]
-              << " radii.txt coupling.txt output.txt [PAMIN PAMAX FRMIN FRMAX]" << std::endl;
+              << " radii.txt coupling.txt output.txt [PAMIN PAMAX FRMIN FRMAX [HEAVY MEDIUM]]"
+              << std::endl;
     return 2;
   }
   double pamin = argc > 4 ? std::atof(argv[4]) : 0.5, pamax = argc > 5 ? std::atof(argv[5]) : 2.0;
@@ -138,7 +153,34 @@ int main(int argc, char** argv) {
   ListCUDADevices();  // CollapseStrength.cu:80-81
   PrintPropertiesOfSpecificDevice(DEVICE);
 
+  auto pa_of = [&](int sid) { return amp[sid % PARES]; };
+  auto f_of = [&](int sid) { return freq[sid / PARES]; };
+  if (argc > 9) {
+    // Split by forcing amplitude over frequency, largest first.
+    const int heavy = std::atoi(argv[8]), medium = std::atoi(argv[9]);
+    if (heavy < 0 || medium < 0 || heavy + medium > NS) {
+      std::cerr << "HEAVY and MEDIUM must be >= 0 with HEAVY + MEDIUM <= " << NS << std::endl;
+      return 2;
+    }
+    std::vector<int> order(NS);
+    for (int i = 0; i < NS; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return pa_of(a) / f_of(a) > pa_of(b) / f_of(b);
+    });
+    std::vector<int> h(order.begin(), order.begin() + heavy);
+    std::vector<int> m(order.begin() + heavy, order.begin() + heavy + medium);
+    Split s(DEVICE, h, m);
+    std::cout << "split solver: " << s.Parts() << " parts" << std::endl;
+    return run(s, radii, M, amp, freq, argv[3]);
+  }
   Solver s(DEVICE);
+  return run(s, radii, M, amp, freq, argv[3]);
+}
+
+template <class S>
+static int run(S& s, const std::vector<double>& radii, const std::vector<double>& M,
+               const std::vector<double>& amp, const std::vector<double>& freq,
+               const char* output) {
   s.SolverOption(SharedCouplingMatrices, 0);
   s.SolverOption(InitialTimeStep, 1e-3);
   s.SolverOption(RelativeTolerance, 0, 1e-10);
@@ -180,7 +222,7 @@ int main(int argc, char** argv) {
   s.SynchroniseSolver();
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
-  std::ofstream out(argv[3]);
+  std::ofstream out(output);
   out.flags(std::ios::scientific);
   for (int sys = 0; sys < NS; sys++)
     for (int c = 0; c < NCOLL; c++) {
@@ -188,18 +230,18 @@ int main(int argc, char** argv) {
       out << sys << ',';
       out.width(17);
       out.precision(10);
-      out << s.GetHost<double>(sys, 0, UnitParameters, 15) << ',';
+      out << s.template GetHost<double>(sys, 0, UnitParameters, 15) << ',';
       out.width(17);
-      out << s.GetHost<double>(sys, 0, UnitParameters, 16) << ',';
+      out << s.template GetHost<double>(sys, 0, UnitParameters, 16) << ',';
       for (int uid = 0; uid < UPS; uid++) {
         out.width(10);
         out.precision(3);
-        out << s.GetHost<double>(sys, uid, UnitAccessories, c) << ',';
+        out << s.template GetHost<double>(sys, uid, UnitAccessories, c) << ',';
       }
       for (int uid = 0; uid < UPS; uid++) {
         out.width(10);
         out.precision(3);
-        out << s.GetHost<double>(sys, uid, UnitAccessories, c + NCOLL) << ',';
+        out << s.template GetHost<double>(sys, uid, UnitAccessories, c + NCOLL) << ',';
       }
       out << '\n';
     }

@@ -18,6 +18,7 @@
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <utility>
 #include <vector>
 #include <type_traits>
 
@@ -603,6 +604,205 @@ class ProblemSolver {
     Check(bex_set_launch_order(handle_, order.empty() ? nullptr : order.data(),
                                static_cast<int>(order.size())),
           "SetLaunchOrder");
+  }
+};
+
+// Split solver (no reference counterpart): one ensemble held by up to three solver objects that
+// run together, each a persistent launch on its own stream — the `heavy` systems one per wave
+// (on the learned BiCG start when a network is given), the `medium` systems one per wave on the
+// plain start, the others two per wave (system-pair layout, 32 units; one per wave otherwise).
+// The heavy and medium waves issue at the top priority. The member functions are
+// ProblemSolver's, with the system indices of the whole ensemble; without a network every
+// result equals ProblemSolver's bit for bit. Print is not provided (each part is an ordinary
+// solver object; Part(k) gives its handle for the C calls).
+template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,
+          int NGP, int NiGP, int NUA, int NiUA, int NSA, int NiSA, int NE, int NDO,
+          Algorithms Algorithm, class Precision>
+class SplitProblemSolver {
+  static_assert(std::is_same<Precision, double>::value || std::is_same<Precision, float>::value,
+                "Precision must be float or double");
+
+ private:
+  struct Piece {
+    bex_solver* h = nullptr;
+    std::vector<int> sids;  // whole-ensemble indices of the part's systems, ascending
+  };
+  std::vector<Piece> parts_;
+  std::vector<int> part_of_, local_of_;
+
+  void Check(int rc, const char* function, const bex_solver* h) const {
+    if (rc == BEX_OK) return;
+    std::cerr << "ERROR: In solver member function " << function << "!" << std::endl;
+    std::cerr << "       " << bex_last_error(h) << std::endl;
+    std::exit(EXIT_FAILURE);
+  }
+  // The part and local index of system sid (the reference's bound check on the system index).
+  std::pair<bex_solver*, int> Route(int sid, const char* function) const {
+    if (sid < 0 || sid >= NS) {
+      std::cerr << "ERROR: In solver member function " << function << "!" << std::endl;
+      std::cerr << "       Option: SystemNumber, index " << sid
+                << " out of range (acceptable index: 0-" << NS - 1 << ")" << std::endl;
+      std::exit(EXIT_FAILURE);
+    }
+    return {parts_[part_of_[sid]].h, local_of_[sid]};
+  }
+
+ public:
+  SplitProblemSolver(int device, const std::vector<int>& heavy, const std::vector<int>& medium,
+                     const float* network = nullptr)
+      : part_of_(NS, 2), local_of_(NS, 0) {
+    std::vector<std::vector<int>> groups(3);
+    for (int k = 0; k < 2; k++) {
+      for (int sid : (k == 0 ? heavy : medium)) {
+        if (sid < 0 || sid >= NS || part_of_[sid] != 2) {
+          std::cerr << "ERROR: SplitProblemSolver: system " << sid
+                    << " is out of range or listed twice" << std::endl;
+          std::exit(EXIT_FAILURE);
+        }
+        part_of_[sid] = k;
+      }
+    }
+    for (int sid = 0; sid < NS; sid++) groups[part_of_[sid]].push_back(sid);
+    std::vector<int> slot(3, -1);
+    for (int k = 0; k < 3; k++) {
+      if (groups[k].empty()) continue;
+      Piece part;
+      part.sids = groups[k];
+      bex_problem_desc d{(int)part.sids.size(), UPS, UD, TPB, SPB, NC, CBW, CCI, NUP, NSP, NGP,
+                         NiGP, NUA, NiUA, NSA, NiSA, NE, NDO, Algorithm};
+      if (bex_create(&d, device, &part.h) != BEX_OK) {
+        std::cerr << "ERROR: creating the solver object failed!" << std::endl;
+        std::cerr << "       " << bex_last_error(nullptr) << std::endl;
+        std::exit(EXIT_FAILURE);
+      }
+      const bool urgent = k < 2;
+      if (urgent) {
+        Check(bex_solver_option(part.h, BEX_PriorityStepThreshold, 1.0), "SplitProblemSolver", part.h);
+      } else if (UPS == 32) {
+        Check(bex_solver_option(part.h, BEX_KernelLayout, BEX_LAYOUT_SYSTEM_PAIR),
+              "SplitProblemSolver", part.h);
+      }
+      if (k == 0 && network) {
+        Check(bex_set_warm_start_network(part.h, network), "SplitProblemSolver", part.h);
+        Check(bex_solver_option(part.h, BEX_WarmStart, 1.0), "SplitProblemSolver", part.h);
+      }
+      slot[k] = (int)parts_.size();
+      for (int i = 0; i < (int)part.sids.size(); i++) local_of_[part.sids[i]] = i;
+      parts_.push_back(part);
+    }
+    for (int sid = 0; sid < NS; sid++) part_of_[sid] = slot[part_of_[sid]];
+  }
+  ~SplitProblemSolver() {
+    for (Piece& p : parts_)
+      if (p.h) bex_destroy(p.h);
+  }
+  SplitProblemSolver(const SplitProblemSolver&) = delete;
+  SplitProblemSolver& operator=(const SplitProblemSolver&) = delete;
+
+  int Parts() const { return (int)parts_.size(); }
+  bex_solver* Part(int k) { return parts_.at(k).h; }
+  const std::vector<int>& PartSystems(int k) const { return parts_.at(k).sids; }
+
+  // Options apply to every part (a KernelLayout given here replaces the parts' own layouts).
+  template <typename T>
+  void SolverOption(ListOfSolverOptions option, T value) {
+    for (Piece& p : parts_) Check(bex_solver_option(p.h, option, (double)value), "SolverOption", p.h);
+  }
+  template <typename T>
+  void SolverOption(ListOfSolverOptions option, int index, T value) {
+    for (Piece& p : parts_)
+      Check(bex_solver_option_indexed(p.h, option, index, (double)value), "SolverOption", p.h);
+  }
+
+  template <typename T>
+  void SetHost(int sid, int uid, ListOfVariables var, int k, T v) {
+    auto r = Route(sid, "SetHost");
+    Check(bex_set_host_unit(r.first, r.second, uid, var, k, (double)v), "SetHost", r.first);
+  }
+  template <typename T>
+  void SetHost(int sid, ListOfVariables var, int k, T v) {
+    auto r = Route(sid, "SetHost");
+    Check(bex_set_host_system(r.first, r.second, var, k, (double)v), "SetHost", r.first);
+  }
+  template <typename T>
+  void SetHost(ListOfVariables var, int k, T v) {
+    for (Piece& p : parts_) Check(bex_set_host_global(p.h, var, k, (double)v), "SetHost", p.h);
+  }
+  template <typename T>
+  void SetHost(int c, ListOfVariables var, int row, int col, T v) {
+    for (Piece& p : parts_)
+      Check(bex_set_host_coupling(p.h, c, var, row, col, (double)v), "SetHost", p.h);
+  }
+  template <typename T>
+  void SetHost(int sid, ListOfVariables var, T v) {
+    auto r = Route(sid, "SetHost");
+    Check(bex_set_host_dense_index(r.first, r.second, var, (double)v), "SetHost", r.first);
+  }
+  template <typename T>
+  void SetHost(int sid, int uid, ListOfVariables var, int comp, int k, T v) {
+    auto r = Route(sid, "SetHost");
+    Check(bex_set_host_dense_state(r.first, r.second, uid, var, comp, k, (double)v), "SetHost",
+          r.first);
+  }
+
+  template <typename T>
+  T GetHost(int sid, int uid, ListOfVariables var, int k) {
+    auto r = Route(sid, "GetHost");
+    double v;
+    Check(bex_get_host_unit(r.first, r.second, uid, var, k, &v), "GetHost", r.first);
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, ListOfVariables var, int k) {
+    auto r = Route(sid, "GetHost");
+    double v;
+    Check(bex_get_host_system(r.first, r.second, var, k, &v), "GetHost", r.first);
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(ListOfVariables var, int k) {
+    double v;
+    Check(bex_get_host_global(parts_[0].h, var, k, &v), "GetHost", parts_[0].h);
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int c, ListOfVariables var, int row, int col) {
+    double v;
+    Check(bex_get_host_coupling(parts_[0].h, c, var, row, col, &v), "GetHost", parts_[0].h);
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, ListOfVariables var) {
+    auto r = Route(sid, "GetHost");
+    double v;
+    Check(bex_get_host_dense_index(r.first, r.second, var, &v), "GetHost", r.first);
+    return (T)v;
+  }
+  template <typename T>
+  T GetHost(int sid, int uid, ListOfVariables var, int comp, int k) {
+    auto r = Route(sid, "GetHost");
+    double v;
+    Check(bex_get_host_dense_state(r.first, r.second, uid, var, comp, k, &v), "GetHost", r.first);
+    return (T)v;
+  }
+
+  void SynchroniseFromHostToDevice(ListOfVariables var) {
+    for (Piece& p : parts_) Check(bex_sync_h2d(p.h, var), "SynchroniseFromHostToDevice", p.h);
+  }
+  void SynchroniseFromDeviceToHost(ListOfVariables var) {
+    for (Piece& p : parts_) Check(bex_sync_d2h(p.h, var), "SynchroniseFromDeviceToHost", p.h);
+  }
+  // Every part's launch, heavy first; asynchronous like ProblemSolver::Solve().
+  void Solve() {
+    for (Piece& p : parts_) Check(bex_solve(p.h), "Solve", p.h);
+  }
+  void SynchroniseDevice() { Check(bex_synchronise_device(parts_[0].h), "SynchroniseDevice", parts_[0].h); }
+  void InsertSynchronisationPoint() {
+    for (Piece& p : parts_)
+      Check(bex_insert_synchronisation_point(p.h), "InsertSynchronisationPoint", p.h);
+  }
+  void SynchroniseSolver() {
+    for (Piece& p : parts_) Check(bex_synchronise_solver(p.h), "SynchroniseSolver", p.h);
   }
 };
 
