@@ -15,10 +15,14 @@ dense output stays in device memory (the PCIe-inclusive rate with each period's 
 copied to the host is reported beside it).
 
 Further legs, each skipped (and said so) when the wall budget would not hold it: the learned
-BiCG warm start, learned + batched, the BiCG residual 1e-13 workload (plain and learned), the
-second period of the headline workload (the steady state, with an attempt cap for the systems
-that reach the Keller-Miksis Mach-1 singularity), the uneven-N pool (20480 systems, a tenth of
-a period) and the N = 256 clusters (4096 systems, a fraction of a period). The CPU baseline times the host restatement on a bounded sample.
+BiCG warm start; learned + batched (three solver objects launched together: the stiffest
+systems on the learned start, the next tier by measured cost one per wave, the rest two per
+wave) and the same split on the plain start (checked bit for bit against the plain leg); the
+BiCG residual 1e-13 workload (plain and learned); the second period of the headline workload
+(the steady state, with an attempt cap for the systems that reach the Keller-Miksis Mach-1
+singularity); the uneven-N pool (20480 systems, a tenth of a period); and the N = 256 clusters
+(4096 systems, a fraction of a period). The CPU baseline times the host restatement on a
+bounded sample. `--leg pool|n256|period2|split` runs one leg alone and prints its record.
 
 Prints one JSON line (metric, value = whole-job system-periods/s of the plain method, roofline
 of the solver kernel on the FP64 pipeline, CPU baseline).
