@@ -106,6 +106,37 @@ def test_config0_eight_periods_digests():
     s.close()
 
 
+def test_config0_sixty_four_periods_digests():
+    """Config 0 as specified, 64 periods (32 transient, the accessory reset, 32 recording), on the
+    46 systems whose 64 periods took the oracle at most 1e6 accepted steps (so the run stays
+    within ~35 s; the costlier rest of config 0, up to 4.6e6 steps, keeps to the 8-period test):
+    each one's state, time, accessories, event counter and counters equal the oracle's, compared
+    through the per-system digests the oracle wrote (tests/golden/make_config0_64periods.py)."""
+    import json
+    import os
+    import sys
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config0_64periods.json")))
+    pick = [k for k, a in enumerate(want["accepted"]) if a <= 1_000_000]
+    assert len(pick) == 46
+    sids = [want["systems"][k] for k in pick]
+    ens = wl.config0().subset(sids)
+    periods = want["transient"] + want["record"]
+    print(f"case: config 0, {ens.NS} systems, {want['transient']}+{want['record']} periods",
+          flush=True)
+    s = run_gpu(ens, periods=periods, reset_after=want["transient"])
+    got = system_digests(ens.NS, ens.N, s.host_view(V.ActualState), s.host_view(V.ActualTime),
+                         s.host_view(V.UnitAccessories), s.host_view(V.IntegerUnitAccessories),
+                         s.stats())
+    bad = [sids[i] for i, k in enumerate(pick) if got[i] != want["digests"][k]]
+    assert not bad, f"systems differing from the oracle: {bad}"
+    s.close()
+
+
 def test_config1_full_period_digests():
     """The headline workload as the benchmark runs it — all 16384 systems of config 1, one
     period from rest, 64 Hermite dense-output points — equals the oracle system by system:

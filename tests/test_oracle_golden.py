@@ -96,6 +96,33 @@ def test_config0_period_digests_reproduce(oracle):
     assert got == [want["digests"][k] for k in pick]
 
 
+def test_config0_64period_digests_reproduce(oracle):
+    """The committed per-system digests of config 0 over 32 + 32 periods (the GPU test's
+    reference) are the oracle's: recomputed here for the two systems with the fewest steps."""
+    import json
+    import os
+    import sys
+
+    from bubbleensemble_implicitextension_amd import workloads as wl
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    from digests import system_digests
+
+    want = json.load(open(os.path.join(here, "config0_64periods.json")))
+    pick = sorted(range(len(want["systems"])), key=lambda k: want["accepted"][k])[:2]
+    ens = wl.config0().subset([want["systems"][k] for k in pick])
+    p = oracle.OracleProblem.from_ensemble(ens)
+    for k in range(want["transient"] + want["record"]):
+        if k == want["transient"]:
+            p.reset_accessories()
+        p.solve(mode=oracle.CANONICAL)
+    got = system_digests(ens.NS, ens.N, p.ActualState, p.ActualTime, p.UnitAccessories,
+                         p.IntegerUnitAccessories, p.stats_table())
+    assert got == [want["digests"][k] for k in pick]
+    assert [int(a) for a in p.stats_table()[:, 0]] == [want["accepted"][k] for k in pick]
+
+
 def test_config4_digest_reproduces(oracle):
     """The committed config-4 digests (the GPU pool test's reference) are the oracle's:
     recomputed here for the first system."""
