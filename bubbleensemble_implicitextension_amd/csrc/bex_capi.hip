@@ -180,7 +180,7 @@ int dense_mode(const bex_solver* s);
 bool kernel_for(int ups, int alg, int layout, int mode, int dense, KernelChoice* k) {
   if (layout == BEX_LAYOUT_SYSTEM_PAIR) {
     if (ups != 32 || mode != bexk::kPlain) return false;
-    *k = choice<bexk::PairLayout32<false>>(alg, dense);
+    *k = choice<bexk::PairLayout32>(alg, dense);
     return true;
   }
   if ((ups == 64 || ups == 128) && layout != BEX_LAYOUT_GROUP) {
@@ -1427,7 +1427,6 @@ const std::vector<PoolVariant>& pool_variants(int n) {
       pool_variant<bexk::GroupLayout<16, 4, true>>("group16 x4 waves")};
   static const std::vector<PoolVariant> v32 = {
       pool_variant<bexk::RolePairLayout32<bexk::kPlain, true>>("rolepair32"),
-      pool_variant<bexk::PairLayout32<true>>("pair32"),
       pool_variant<bexk::GroupLayout<32, 1, true>>("group32 x1 wave"),
       pool_variant<bexk::GroupLayout<32, 2, true>>("group32 x2 waves")};
   static const std::vector<PoolVariant> v64 = {

@@ -12,6 +12,9 @@
 //                     two v_permlane16_swap, no LDS traffic. Dot products pair the halves with
 //                     v_permlane32_swap and reduce with the reference's halving tree through
 //                     permlane16/row_shl DPP; every system-level decision is wave-uniform.
+//   PairLayout32      N = 32, two systems per wave, lane = unit; every lane holds row i of M and
+//                     of M^T, so the products run as in the role pair for both systems at once
+//                     and everything else is issued once for the two.
 //   GroupLayout<N>    N = 8, 16, 64: lane = unit, a wave holds 64/N systems, M and M^T live in
 //                     LDS (one copy per 4-wave workgroup) and the search directions are
 //                     exchanged through per-system LDS slots.
@@ -1308,8 +1311,8 @@ struct RolePairLayout32 {
 
 // =========================================================================================
 // PairLayout32: two N = 32 systems per wave, lane = unit. Lanes 0-31 serve one system, lanes
-// 32-63 the next, and every lane holds row i of M and row i of M^T in registers (128 VGPRs: the
-// shared coupling matrix, or with PS each system's own). A product M z runs as in the role
+// 32-63 the next, and every lane holds row i of M and row i of M^T in registers (128 VGPRs; the
+// coupling matrix all systems share, as in the reference). A product M z runs as in the role
 // pair (two v_permlane16_swap and 32 v_fmac_f64 with a DPP row_newbcast operand, which never
 // cross a 32-lane half), so one pass computes M z for both systems and a second pass M^T w; the
 // halving trees, the divisions, the right-hand side, the controller and the event logic run once
@@ -1318,7 +1321,6 @@ struct RolePairLayout32 {
 // through the exec mask, so a wave costs the slower of its two systems (systems at adjacent
 // grid points cost within about 1 % of each other). The BiCG arithmetic is group_coupling's,
 // i.e. the canonical order; the unit parameters live in LDS ([lane][NUP]).
-template <bool PS = false>
 struct PairLayout32 {
   static constexpr int kUnits = 32;
   static constexpr int kSystemsPerWave = 2;
@@ -1327,7 +1329,7 @@ struct PairLayout32 {
   static constexpr int kSystemsPerBlock = 2;
   static constexpr int kWavesPerEU = 2;  // 256 VGPRs: the two matrix rows take half of them
   static constexpr int kMode = kPlain;
-  static constexpr bool kPerSystemM = PS;
+  static constexpr bool kPerSystemM = false;
   static constexpr int kItemsPerBlock = 1;
   static constexpr int kSystemsPerItem = 2;
   static constexpr bool kUnitParamsInLds = true;
@@ -1366,19 +1368,10 @@ struct PairLayout32 {
     const int i = lane & 31;
     if (i < BEX_NSTAT) stat[i] = 0;
     if (s >= 0 && s < a.NS) {
-      if (PS) {
-        const double* m = a.CouplingMatrix + (size_t)s * 1024;
 #pragma unroll
-        for (int j = 0; j < 32; j++) {
-          mrow[j] = m[i + j * 32];
-          mtrow[j] = m[j + i * 32];
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < 32; j++) {
-          mrow[j] = a.CouplingMatrix[i + j * 32];
-          mtrow[j] = a.CouplingMatrixT[i + j * 32];
-        }
+      for (int j = 0; j < 32; j++) {
+        mrow[j] = a.CouplingMatrix[i + j * 32];
+        mtrow[j] = a.CouplingMatrixT[i + j * 32];
       }
     }
     wave_sync();
