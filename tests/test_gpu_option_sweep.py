@@ -23,13 +23,12 @@ from gpu_helpers import assert_bitwise, compare_solver_to_oracle, run_gpu, run_o
 SEEDS = [101, 202, 303, 404, 505, 606]
 
 
-def draw_case(seed):
-    """A seeded ensemble and option set (host-side only)."""
+def draw_case(seed, n=32):
+    """A seeded ensemble of n-unit systems and an option set (host-side only)."""
     rng = np.random.default_rng(seed)
     ns = int(rng.integers(3, 10))
     pa = rng.uniform(0.5, 1.6, size=ns)
     f = rng.uniform(40.0, 200.0, size=ns)
-    n = 32
     _, re_um, d = wl.line_geometry(n, seed=seed)
     params = wl.unit_parameters(pa[:, None], f[:, None], re_um[None, :])
     state0 = np.zeros((ns, n, 2))
@@ -92,3 +91,29 @@ def test_random_options_every_layout_and_launch(oracle, seed):
             assert_bitwise("DenseState", s.host_view(V.DenseState)[: n * TLT * 2],
                            p.DenseOutputStates[: n * TLT * 2])
             s.close()
+
+
+# Other unit counts and their layouts: lane groups (M in LDS) for 8, 16 and 64 units, role pairs
+# for 64 (two waves), the eight-wave chain layout for 128, the streamed wide layout for 96 and
+# 256 (a reference fold for 96). Shorter time domains keep the oracle's part to seconds.
+OTHER = [(8, LAYOUT_GROUP, 0.5), (16, LAYOUT_GROUP, 0.5), (64, LAYOUT_GROUP, 0.2),
+         (64, LAYOUT_ROLE_PAIR, 0.2), (96, LAYOUT_GROUP, 0.1), (128, LAYOUT_ROLE_PAIR, 0.1),
+         (256, LAYOUT_GROUP, 0.02)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,layout,end", OTHER)
+@pytest.mark.parametrize("seed", [707, 808])
+def test_random_options_other_unit_counts(oracle, seed, n, layout, end):
+    ens, alg, opts, _ = draw_case(seed, n)
+    ndo = 40
+    print(f"case: seed {seed}, N {n}, layout {layout}, NS {ens.NS}, algorithm {alg}, end {end}",
+          flush=True)
+    p = run_oracle(oracle, ens, periods=2, algorithm=alg, options=opts, ndo=ndo, end_time=end)
+    s = run_gpu(ens, periods=2, algorithm=alg, options=opts, ndo=ndo, end_time=end, layout=layout)
+    compare_solver_to_oracle(s, p)
+    n_out = int(p.DenseOutputIndex.max())
+    assert_bitwise("DenseIndex", s.host_view(V.DenseIndex), p.DenseOutputIndex)
+    assert_bitwise("DenseTime", s.host_view(V.DenseTime)[: n_out * ens.NS],
+                   p.DenseOutputTimeInstances[: n_out * ens.NS])
+    s.close()
