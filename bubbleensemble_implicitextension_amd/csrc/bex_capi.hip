@@ -74,6 +74,7 @@ struct bex_solver {
   int BiCGMaxIterations = 19;
   int KernelLayout = BEX_LAYOUT_AUTO;
   int PersistentLaunch = 1;  // 1: persistent waves over a device work queue (bex_solve)
+  int PersistentWorkgroups = 0;  // > 0: cap on the persistent launch's workgroups
   unsigned long long* d_queue = nullptr;  // its counter
   int* d_order = nullptr;                 // hand-out order of the systems (nullptr: index order)
   int cu_count = 0;
@@ -701,6 +702,11 @@ int bex_solver_option(bex_solver* s, int option, double v) {
       s->MaxStepAttempts = (long long)v;
       break;
     case BEX_PersistentLaunch: s->PersistentLaunch = (v != 0.0) ? 1 : 0; break;
+    case BEX_PersistentWorkgroups:
+      if (!(v >= 0.0) || v > 2.0e9)
+        return fail(s, BEX_E_ARGUMENT, "SolverOption: PersistentWorkgroups must be >= 0");
+      s->PersistentWorkgroups = (int)v;
+      break;
     case BEX_KernelLayout: {
       KernelChoice kc;
       if (!kernel_for(s->d.UPS, s->d.Algorithm, (int)v, bexk::kPlain, bexk::kDenseNone, &kc))
@@ -1035,6 +1041,49 @@ int bex_sync_d2h(bex_solver* s, int var) {
 
 static int pool_class_solve(bex_solver* s);  // pool section below
 
+// Systems a persistent launch hands out (with a hand-out order every system is listed; without
+// one, systems 0 .. nsys-1) and the work items they make (systems_per_item systems each).
+static void persistent_items(const bex_solver* s, const KernelChoice& kc, int nsys,
+                             long long* count, long long* nitems) {
+  *count = s->d_order ? (long long)s->d.NS : (long long)nsys;
+  *nitems = (*count + kc.systems_per_item - 1) / kc.systems_per_item;
+}
+
+// Workgroups of a persistent launch of kernel kc over nitems work items: as many as fit on the
+// device, at most PersistentWorkgroups when that is set, and no more than the items need.
+static int persistent_blocks(bex_solver* s, const KernelChoice& kc, long long nitems,
+                             long long* blocks) {
+  int per_cu = 0;
+  BEX_HIP(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per_cu, reinterpret_cast<const void*>(kc.queue), kc.block, kc.lds));
+  if (per_cu < 1) return fail(s, BEX_E_UNSUPPORTED, "Solve: the kernel does not fit on a CU");
+  long long b = (long long)s->cu_count * per_cu;
+  if (s->PersistentWorkgroups > 0 && b > s->PersistentWorkgroups) b = s->PersistentWorkgroups;
+  const long long need = (nitems + kc.items_per_block - 1) / kc.items_per_block;
+  *blocks = b < need ? b : need;
+  return BEX_OK;
+}
+
+int bex_persistent_grid(bex_solver* s, int* workgroups, int* waves_per_workgroup) {
+  if (!s || !workgroups || !waves_per_workgroup) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  KernelChoice kc;
+  const int mode = s->WarmStart ? bexk::kLearned : (s->d_samples ? bexk::kSampling : bexk::kPlain);
+  if (s->pool || !s->PersistentLaunch ||
+      !kernel_for(s->d.UPS, s->d.Algorithm, s->KernelLayout, mode, dense_mode(s), &kc))
+    return fail(s, BEX_E_UNSUPPORTED, "bex_persistent_grid: no persistent kernel for this solver");
+  const int nsys = s->ActiveSystems < s->d.NS ? s->ActiveSystems : s->d.NS;
+  long long count = 0, nitems = 0, blocks = 0;
+  persistent_items(s, kc, nsys, &count, &nitems);
+  if (nsys > 0) {
+    const int rc = persistent_blocks(s, kc, nitems, &blocks);
+    if (rc != BEX_OK) return rc;
+  }
+  *workgroups = (int)blocks;
+  *waves_per_workgroup = kc.block / 64;
+  return BEX_OK;
+}
+
 int bex_solve(bex_solver* s) {
   if (!s) return BEX_E_ARGUMENT;
   BEX_HIP(s, hipSetDevice(s->device));
@@ -1055,19 +1104,13 @@ int bex_solve(bex_solver* s) {
   if (s->PersistentLaunch) {
     // As many workgroups as fit on the device at once, each taking work items from the queue
     // (the counter is cleared by a kernel, so the sequence can be captured into a graph).
-    // With a hand-out order every system is listed; without one, systems 0 .. nsys-1.
     if (kc.lds > 65536)
       BEX_HIP(s, hipFuncSetAttribute(reinterpret_cast<const void*>(kc.queue),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kc.lds));
-    const long long count = s->d_order ? (long long)s->d.NS : (long long)nsys;
-    const long long nitems = (count + kc.systems_per_item - 1) / kc.systems_per_item;
-    int per_cu = 0;
-    BEX_HIP(s, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                   &per_cu, reinterpret_cast<const void*>(kc.queue), kc.block, kc.lds));
-    if (per_cu < 1) return fail(s, BEX_E_UNSUPPORTED, "Solve: the kernel does not fit on a CU");
-    long long blocks = (long long)s->cu_count * per_cu;
-    const long long need = (nitems + kc.items_per_block - 1) / kc.items_per_block;
-    if (blocks > need) blocks = need;
+    long long count = 0, nitems = 0, blocks = 0;
+    persistent_items(s, kc, nsys, &count, &nitems);
+    const int rc = persistent_blocks(s, kc, nitems, &blocks);
+    if (rc != BEX_OK) return rc;
     hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, s->stream, s->d_queue);
     hipLaunchKernelGGL(kc.queue, dim3((unsigned)blocks), dim3(kc.block), kc.lds, s->stream, a,
                        (const int*)s->d_order, count, nitems, s->d_queue);

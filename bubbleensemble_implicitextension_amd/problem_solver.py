@@ -67,6 +67,7 @@ class ListOfSolverOptions(enum.IntEnum):  # Interface.cuh:55-68, then additions
     DenseOutputInterpolation = 20
     MaximumStepAttempts = 21
     PersistentLaunch = 22
+    PersistentWorkgroups = 23
 
 
 LAYOUT_AUTO, LAYOUT_GROUP, LAYOUT_ROLE_PAIR, LAYOUT_SYSTEM_PAIR = 0, 1, 2, 3
@@ -254,6 +255,12 @@ class ProblemSolver:
         self._check(self._lib.bex_checkpoint_restore(self._h))
 
     # -- persistent launch (no reference counterpart) ------------------------------------
+    def persistent_grid(self):
+        """(workgroups, waves per workgroup) of the persistent launch the next Solve() starts."""
+        wg, w = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.bex_persistent_grid(self._h, ctypes.byref(wg), ctypes.byref(w)))
+        return wg.value, w.value
+
     def set_launch_order(self, order):
         """Order in which Solve()'s persistent launch hands out the systems: every system
         index once, longest expected run first; None returns to index order. Results do not

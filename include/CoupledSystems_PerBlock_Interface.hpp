@@ -71,7 +71,8 @@ enum ListOfSolverOptions {  // Interface.cuh:55-68, then the BiCG constants
   WarmStart,
   DenseOutputInterpolation,
   MaximumStepAttempts,
-  PersistentLaunch
+  PersistentLaunch,
+  PersistentWorkgroups
 };
 
 // --- Names of the enumerators (Interface.cuh:70-71, 2035-2106) ----------------------------
@@ -85,7 +86,7 @@ inline std::string SolverOptionsToString(ListOfSolverOptions Option) {
       "SharedGlobalVariables", "SharedCouplingMatrices", "BiCGRhoTolerance",
       "BiCGResidualTolerance", "BiCGMaxIterations",      "KernelLayout",
       "PriorityStepThreshold", "WarmStart",              "DenseOutputInterpolation",
-      "MaximumStepAttempts",   "PersistentLaunch"};
+      "MaximumStepAttempts",   "PersistentLaunch",       "PersistentWorkgroups"};
   const int i = static_cast<int>(Option) - static_cast<int>(InitialTimeStep);
   const int n = static_cast<int>(sizeof(names) / sizeof(names[0]));
   return (i >= 0 && i < n) ? names[i] : "Non-existent solver option!";

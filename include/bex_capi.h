@@ -105,7 +105,11 @@ enum bex_option {
    * systems, dispatched by the hardware in index order, as the reference launches
    * (Interface.cuh:1949-1956). The results are the same; the queue keeps every slot busy when
    * the systems' run times differ. No reference counterpart. */
-  BEX_PersistentLaunch
+  BEX_PersistentLaunch,
+  /* > 0: at most this many workgroups in the persistent launch (0, the default: as many as fit
+   * on the device). Leaves room on the device for other work that runs at the same time. No
+   * reference counterpart. */
+  BEX_PersistentWorkgroups
 };
 
 #define BEX_LAYOUT_AUTO 0      /* role pairs for 32, 64 and 128 units, lane groups otherwise */
@@ -201,6 +205,12 @@ int bex_clear_stop(bex_solver* s);
  * per wave, consecutive entries share a wave (for the system pair: entries 2k and 2k + 1), so
  * systems of similar cost should be neighbours. Results do not depend on it. */
 int bex_set_launch_order(bex_solver* s, const int* order, int n);
+
+/* Workgroups of the persistent launch that the next Solve() would start (as many as fit on the
+ * device, or PersistentWorkgroups when that is smaller, and no more than the systems need; 0
+ * without systems to solve), and the waves each has; BEX_E_UNSUPPORTED with PersistentLaunch 0
+ * or per-system coupling matrices. No reference counterpart. */
+int bex_persistent_grid(bex_solver* s, int* workgroups, int* waves_per_workgroup);
 
 /* Per-system counters [NS][32] accumulated over Solve() calls (layout in bex_types.h);
  * copies device counters to `out` (synchronous). No reference counterpart. */

@@ -145,6 +145,35 @@ def test_error_conventions():
     s.close()
 
 
+def test_persistent_grid_query_and_cap():
+    """The persistent launch's grid: one wave per work item up to what fits on the device (a
+    role-pair item is one system, a system-pair item two), at most PersistentWorkgroups when
+    set; a negative cap is rejected, and the query fails for the hardware-dispatched launch."""
+    O = ps.ListOfSolverOptions
+    ens = wl.small_ensemble(n=32, ns=9, seed=4)
+    assert ens.NS == 9
+    s = collapse.make_solver(ens)
+    collapse.fill(s, ens)
+    s.SolverOption(O.KernelLayout, ps.LAYOUT_ROLE_PAIR)
+    assert s.persistent_grid() == (9, 1)
+    s.SolverOption(O.KernelLayout, ps.LAYOUT_SYSTEM_PAIR)
+    assert s.persistent_grid() == (5, 1)       # five pairs, the last one with one system
+    s.SolverOption(O.PersistentWorkgroups, 3)
+    assert s.persistent_grid() == (3, 1)
+    s.SolverOption(O.KernelLayout, ps.LAYOUT_ROLE_PAIR)
+    assert s.persistent_grid() == (3, 1)
+    s.SolverOption(O.ActiveSystems, 2)
+    assert s.persistent_grid() == (2, 1)       # two role-pair systems
+    s.SolverOption(O.KernelLayout, ps.LAYOUT_SYSTEM_PAIR)
+    assert s.persistent_grid() == (1, 1)       # one system pair
+    with pytest.raises(BexError, match="PersistentWorkgroups"):
+        s.SolverOption(O.PersistentWorkgroups, -1)
+    s.SolverOption(O.PersistentLaunch, 0)
+    with pytest.raises(BexError, match="no persistent kernel"):
+        s.persistent_grid()
+    s.close()
+
+
 def test_device_helpers():
     """ListCUDADevices / SelectDeviceByClosestRevision / PrintPropertiesOfSpecificDevice
     (Interface.cuh:259-345) report the HIP device."""

@@ -233,22 +233,28 @@ def test_config1_full_size_layouts_and_reruns_agree():
     systems per wave) agree bit for bit in state, accessories, event counts and counters over a
     fifth of a period, and so does the system-pair kernel (two systems per wave, rows of M and
     M^T in registers); a second role-pair launch repeats the first bit for bit, and so do the
-    hardware-dispatched launch (one workgroup per system) and a persistent launch that hands
-    the systems out in reverse order."""
+    hardware-dispatched launch (one workgroup per system), a persistent launch that hands
+    the systems out in reverse order and one capped at 300 workgroups (about 55 systems per
+    wave)."""
     from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
 
     big = wl.config1()
     print("case: config 1, 16384 systems x N=32, tau in [0, 0.2]: role pair (persistent "
-          "twice, dispatched, reverse order), lane group, system pair", flush=True)
+          "twice, dispatched, reverse order, 300 workgroups), lane group, system pair",
+          flush=True)
     runs = []
     for layout, how in ((LAYOUT_ROLE_PAIR, None), (LAYOUT_ROLE_PAIR, None),
                         (LAYOUT_ROLE_PAIR, "dispatched"), (LAYOUT_ROLE_PAIR, "reverse"),
-                        (LAYOUT_GROUP, None), (LAYOUT_SYSTEM_PAIR, None)):
+                        (LAYOUT_ROLE_PAIR, "capped"), (LAYOUT_GROUP, None),
+                        (LAYOUT_SYSTEM_PAIR, None)):
         def setup(s):
             if how == "dispatched":
                 s.SolverOption(O.PersistentLaunch, 0)
             elif how == "reverse":
                 s.set_launch_order(np.arange(big.NS)[::-1])
+            elif how == "capped":
+                s.SolverOption(O.PersistentWorkgroups, 300)
+                assert s.persistent_grid() == (300, 1)
         s = run_gpu(big, periods=1, end_time=0.2, layout=layout, setup=setup)
         runs.append([s.state().copy(), s.accessories().copy(), s.event_counts().copy(),
                      s.stats().copy()])
