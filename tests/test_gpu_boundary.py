@@ -45,11 +45,13 @@ def with_coupling(ens, m):
                        m, ens.state0, dict(ens.options), ens.periods)
 
 
-def solve_stored(ens, cbw, cci, values):
+def solve_stored(ens, cbw, cci, values, layout=ps.LAYOUT_AUTO):
     """A solver with the given storage scheme, coupling 0 set element by element through
     SetHost from `values` (the dense matrix whose stored elements are written); returns the
-    solver after one Solve() and the set of (row, col) that SetHost accepted."""
+    solver after one Solve() with the given kernel layout and the set of (row, col) that
+    SetHost accepted."""
     s = collapse.make_solver(ens, CBW=cbw, CCI=cci)
+    s.SolverOption(ps.ListOfSolverOptions.KernelLayout, layout)
     n = ens.N
     # Everything but the coupling (fill leaves per-system matrices to the caller).
     collapse.fill(s, with_coupling(ens, np.zeros((ens.NS, n, n))))
@@ -113,6 +115,12 @@ def test_coupling_storage_schemes(oracle, cbw, cci):
     assert_bitwise("stats", s.stats(), full.stats())
     p = run_oracle(oracle, with_coupling(ens, expect), periods=1, end_time=END)
     compare_solver_to_oracle(s, p)
+    # The system-pair layout reads the expanded matrix and its transpose the same way.
+    sp, _ = solve_stored(ens, cbw, cci, values, layout=ps.LAYOUT_SYSTEM_PAIR)
+    for var in (V.ActualState, V.ActualTime, V.UnitAccessories, V.IntegerUnitAccessories):
+        assert_bitwise(var.name, sp.host_view(var), full.host_view(var))
+    assert_bitwise("stats", sp.stats(), full.stats())
+    sp.close()
     s.close()
     full.close()
 
