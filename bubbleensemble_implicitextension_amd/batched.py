@@ -1,8 +1,9 @@
 """Learned + batched solving of one ensemble (no reference counterpart).
 
-The costliest systems (by a cost proxy that needs no run) take the learned BiCG warm start, the
-others the plain start; both parts are ordinary solver objects on their own streams, launched
-together. This module needs no torch: the network is a trained parameter file.
+The stiffest systems (by a cost proxy that needs no run) take the learned BiCG warm start, the
+others the plain start, split by cost into a tier of one system per wave and a bulk of two per
+wave; the parts are ordinary solver objects on their own streams, launched together. This
+module needs no torch: the network is a trained parameter file.
 """
 from __future__ import annotations
 
