@@ -1147,7 +1147,11 @@ int bex_set_launch_order(bex_solver* s, const int* order, int n) {
   }
   BEX_HIP(s, hipStreamSynchronize(s->stream));
   if (!s->d_order) BEX_HIP(s, hipMalloc((void**)&s->d_order, sizeof(int) * (size_t)n));
-  BEX_HIP(s, hipMemcpy(s->d_order, order, sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  // On the solver's stream, which the next Solve() follows; complete before `order` is
+  // returned to the caller.
+  BEX_HIP(s, hipMemcpyAsync(s->d_order, order, sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
+                            s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
   return BEX_OK;
 }
 
@@ -1364,7 +1368,8 @@ int bex_mlp_predict_mfma(const float* rows, int stride, long long n, const float
   if (hipMalloc((void**)&d, sizeof(BexMlpDevice)) != hipSuccess) return BEX_E_MEMORY;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   int rc = BEX_OK;
-  if (hipMemcpy(d, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice) != hipSuccess) rc = BEX_E_HIP;
+  if (hipMemcpyAsync(d, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = BEX_E_HIP;
   if (rc == BEX_OK) {
     hipLaunchKernelGGL(mlp_mfma_kernel, dim3((unsigned)nb), dim3(64), 0, st, rows, stride, n, d, out);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = BEX_E_HIP;
@@ -1382,9 +1387,11 @@ int bex_set_warm_start_network(bex_solver* s, const float* params) {
   BexMlpDevice host;
   bex_mlp_pack_device(params, &host);
   if (!s->d_net) BEX_HIP(s, hipMalloc((void**)&s->d_net, sizeof(BexMlpDevice)));
-  // Synchronous: the host staging copy goes out of scope on return.
+  // On the solver's stream, complete before return: the host staging copy goes out of scope.
   BEX_HIP(s, hipStreamSynchronize(s->stream));
-  BEX_HIP(s, hipMemcpy(s->d_net, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice));
+  BEX_HIP(s, hipMemcpyAsync(s->d_net, &host, sizeof(BexMlpDevice), hipMemcpyHostToDevice,
+                            s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
   return BEX_OK;
 }
 
@@ -1709,6 +1716,9 @@ int bex_pool_create(int nclass, const int* units, const int* systems, int device
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds) != hipSuccess)
         return bail(BEX_E_HIP, "bex_pool_create: LDS attribute failed");
   }
+  // The matrix clears and order uploads above ran on the null stream, which the pool's
+  // non-blocking streams do not wait for.
+  if (hipDeviceSynchronize() != hipSuccess) return bail(BEX_E_HIP, "bex_pool_create: synchronisation failed");
   *out = p;
   return BEX_OK;
 }
@@ -1747,19 +1757,25 @@ int bex_pool_set_couplings(bex_pool* p, int c, int first, int count, const doubl
     return pfail(p, BEX_E_BOUND, "bex_pool_set_couplings: system range out of bounds");
   BEX_PHIP(p, hipSetDevice(p->device));
   BEX_PHIP(p, hipStreamSynchronize(k.s->stream));
+  BEX_PHIP(p, hipStreamSynchronize(p->stream));
   const size_t nn = (size_t)k.N * k.N;
-  BEX_PHIP(p, hipMemcpy(k.s->d_CouplingSys + (size_t)first * nn, M, sizeof(double) * nn * count,
-                        hipMemcpyHostToDevice));
-  if (k.s->d_CouplingSysT) {  // transposes for the streamed layouts
-    std::vector<double> t(nn);
+  // On the class solver's stream, complete before return (the pool's graph and the class's own
+  // Solve() come after it).
+  BEX_PHIP(p, hipMemcpyAsync(k.s->d_CouplingSys + (size_t)first * nn, M,
+                             sizeof(double) * nn * count, hipMemcpyHostToDevice, k.s->stream));
+  std::vector<double> t;
+  if (k.s->d_CouplingSysT && count > 0) {  // transposes for the streamed layouts
+    t.resize(nn * (size_t)count);
     for (int q = 0; q < count; q++) {
       const double* src = M + (size_t)q * nn;
+      double* dst = t.data() + (size_t)q * nn;
       for (int r = 0; r < k.N; r++)
-        for (int c2 = 0; c2 < k.N; c2++) t[(size_t)c2 + (size_t)r * k.N] = src[(size_t)r + (size_t)c2 * k.N];
-      BEX_PHIP(p, hipMemcpy(k.s->d_CouplingSysT + (size_t)(first + q) * nn, t.data(),
-                            sizeof(double) * nn, hipMemcpyHostToDevice));
+        for (int c2 = 0; c2 < k.N; c2++) dst[(size_t)c2 + (size_t)r * k.N] = src[(size_t)r + (size_t)c2 * k.N];
     }
+    BEX_PHIP(p, hipMemcpyAsync(k.s->d_CouplingSysT + (size_t)first * nn, t.data(),
+                               sizeof(double) * nn * count, hipMemcpyHostToDevice, k.s->stream));
   }
+  BEX_PHIP(p, hipStreamSynchronize(k.s->stream));
   return BEX_OK;
 }
 
@@ -1774,7 +1790,10 @@ int bex_pool_set_order(bex_pool* p, int c, const int* order) {
   }
   BEX_PHIP(p, hipSetDevice(p->device));
   BEX_PHIP(p, hipStreamSynchronize(p->stream));
-  BEX_PHIP(p, hipMemcpy(k.d_order, order, sizeof(int) * (size_t)k.NS, hipMemcpyHostToDevice));
+  // On the pool's stream, which the next period's graph follows; complete before return.
+  BEX_PHIP(p, hipMemcpyAsync(k.d_order, order, sizeof(int) * (size_t)k.NS, hipMemcpyHostToDevice,
+                             p->stream));
+  BEX_PHIP(p, hipStreamSynchronize(p->stream));
   return BEX_OK;
 }
 
