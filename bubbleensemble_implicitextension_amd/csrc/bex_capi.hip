@@ -1897,15 +1897,22 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
       }
       return rc == BEX_OK;
     };
+    // Copies on the pool's stream, which the probe launches follow, each complete on return
+    // (the snapshot, the probe time domain and the counters are read or reused right after).
+    auto copy = [&](void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+      hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, p->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+      return e;
+    };
     // Snapshot of the class's device state and counters.
     for (size_t b = 0; b < nkeep && rc == BEX_OK; b++)
       if (check(hipMalloc(&saved[b], keep[b]->bytes()), "hipMalloc"))
-        check(hipMemcpy(saved[b], keep[b]->d, keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
+        check(copy(saved[b], keep[b]->d, keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
     if (check(hipMalloc((void**)&stats_saved, sbytes), "hipMalloc"))
-      check(hipMemcpy(stats_saved, s->d_stats, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+      check(copy(stats_saved, s->d_stats, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
     // Probe time domain: [TD0, TD0 + f (TD1 - TD0)] from the device copy.
     std::vector<double> td(2 * (size_t)k.NS);
-    check(hipMemcpy(td.data(), s->TimeDomain.d, sizeof(double) * td.size(), hipMemcpyDeviceToHost),
+    check(copy(td.data(), s->TimeDomain.d, sizeof(double) * td.size(), hipMemcpyDeviceToHost),
           "hipMemcpy");
     for (int i = 0; i < k.NS; i++) td[k.NS + i] = td[i] + probe_fraction * (td[k.NS + i] - td[i]);
     check(hipEventCreate(&t0), "hipEventCreate");
@@ -1915,8 +1922,8 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
     float best_ms = 0.0f;
     for (int v = 0; v < nv && rc == BEX_OK; v++) {
       for (size_t b = 0; b < nkeep; b++)
-        check(hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
-      check(hipMemcpy(s->TimeDomain.d, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice),
+        check(copy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice), "hipMemcpy");
+      check(copy(s->TimeDomain.d, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice),
             "hipMemcpy");
       if (rc != BEX_OK) break;
       k.variant = v;
@@ -1944,8 +1951,8 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
     std::vector<int> order;
     if (rc == BEX_OK && stats_saved) {
       std::vector<long long> now((size_t)k.NS * BEX_NSTAT), before((size_t)k.NS * BEX_NSTAT);
-      if (check(hipMemcpy(now.data(), s->d_stats, sbytes, hipMemcpyDeviceToHost), "hipMemcpy") &&
-          check(hipMemcpy(before.data(), stats_saved, sbytes, hipMemcpyDeviceToHost), "hipMemcpy")) {
+      if (check(copy(now.data(), s->d_stats, sbytes, hipMemcpyDeviceToHost), "hipMemcpy") &&
+          check(copy(before.data(), stats_saved, sbytes, hipMemcpyDeviceToHost), "hipMemcpy")) {
         std::vector<long long> work(k.NS);
         for (int i = 0; i < k.NS; i++) {
           const size_t b = (size_t)i * BEX_NSTAT;
@@ -1959,9 +1966,9 @@ int bex_pool_autotune(bex_pool* p, double probe_fraction, double* times_ms) {
     }
     // Restore the snapshot whatever happened, then release it.
     for (size_t b = 0; b < nkeep; b++)
-      if (saved[b]) check(hipMemcpy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice),
+      if (saved[b]) check(copy(keep[b]->d, saved[b], keep[b]->bytes(), hipMemcpyDeviceToDevice),
                           "hipMemcpy");
-    if (stats_saved) check(hipMemcpy(s->d_stats, stats_saved, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+    if (stats_saved) check(copy(s->d_stats, stats_saved, sbytes, hipMemcpyDeviceToDevice), "hipMemcpy");
     for (void* d : saved)
       if (d) hipFree(d);
     if (stats_saved) hipFree(stats_saved);
