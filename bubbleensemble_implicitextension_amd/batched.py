@@ -114,6 +114,11 @@ class LearnedBatchedSolver:
                 s.SolverOption(O.PriorityStepThreshold, 1)
             if options:
                 collapse.apply_options(s, options)
+                if "ActiveSystems" in options:
+                    # Systems 0 .. n-1 of the ensemble are the part's first local ones (each
+                    # part keeps its systems in ascending order).
+                    s.SolverOption(O.ActiveSystems,
+                                   int(np.count_nonzero(sids < int(options["ActiveSystems"]))))
             if net is not None:
                 s.set_warm_start_network(net)
                 s.SolverOption(O.WarmStart, 1)

@@ -704,10 +704,20 @@ class SplitProblemSolver {
   bex_solver* Part(int k) { return parts_.at(k).h; }
   const std::vector<int>& PartSystems(int k) const { return parts_.at(k).sids; }
 
-  // Options apply to every part (a KernelLayout given here replaces the parts' own layouts).
+  // Options apply to every part (a KernelLayout given here replaces the parts' own layouts);
+  // ActiveSystems n keeps systems 0 .. n-1 of the ensemble, i.e. each part's systems below n,
+  // which are its first local ones.
   template <typename T>
   void SolverOption(ListOfSolverOptions option, T value) {
-    for (Piece& p : parts_) Check(bex_solver_option(p.h, option, (double)value), "SolverOption", p.h);
+    for (Piece& p : parts_) {
+      double v = (double)value;
+      if (option == ActiveSystems) {
+        int m = 0;
+        while (m < (int)p.sids.size() && p.sids[m] < (int)value) m++;
+        v = (double)m;
+      }
+      Check(bex_solver_option(p.h, option, v), "SolverOption", p.h);
+    }
   }
   template <typename T>
   void SolverOption(ListOfSolverOptions option, int index, T value) {

@@ -350,6 +350,33 @@ def test_batched_split_bit_identical_to_plain(oracle):
     assert_bitwise("stats", got["stats"], p.stats_table())
 
 
+def test_batched_split_active_systems(oracle):
+    """ActiveSystems on the split: the ensemble's systems 0 .. 4 of 9 are integrated, wherever
+    the split puts them (each part gets the count of its systems below 5), the rest stay
+    untouched, as in the oracle."""
+    from bubbleensemble_implicitextension_amd import batched
+    from gpu_helpers import assert_bitwise
+
+    ens = wl.small_ensemble(n=32, ns=9, seed=21)
+    cost = np.array([3.0, 9.0, 1.0, 7.0, 2.0, 8.0, 6.0, 0.0, 5.0])
+    opts = {"ActiveSystems": 5}
+    print("case: split of 9 systems, heavy 2, medium 2, ActiveSystems 5", flush=True)
+    lb = batched.LearnedBatchedSolver(ens, None, heavy=2, medium=2, cost=cost, end_time=0.3,
+                                      options=opts)
+    assert len(lb.parts) == 3
+    lb.Solve()
+    lb.synchronise()
+    got = lb.results()
+    lb.close()
+    p = run_oracle(oracle, ens, periods=1, end_time=0.3, options=opts)
+    NS, N = ens.NS, ens.N
+    assert got["stats"][:5, 0].min() > 0 and got["stats"][5:].max() == 0
+    assert_bitwise("state", got["state"], np.moveaxis(p.ActualState.reshape(2, NS, N), 0, -1))
+    assert_bitwise("acc", got["acc"], np.moveaxis(p.UnitAccessories.reshape(-1, NS, N), 0, -1))
+    np.testing.assert_array_equal(got["events"], p.event_counts())
+    assert_bitwise("stats", got["stats"], p.stats_table())
+
+
 @pytest.mark.parametrize("layout,persistent", [(LAYOUT_ROLE_PAIR, 1), (LAYOUT_ROLE_PAIR, 0),
                                                (LAYOUT_SYSTEM_PAIR, 1), (LAYOUT_SYSTEM_PAIR, 0),
                                                (LAYOUT_GROUP, 1)])
