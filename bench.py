@@ -178,11 +178,12 @@ def cpu_baseline(sample=2048, seed=7):
                                     f"({len(one_sids)} systems), 1 thread, {dt1:.1f} s"}
 
 
-def measured_traffic():
-    """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
-    of this bench command (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's
-    reading for a plain device copy of known size). None when the summary is absent."""
-    path = os.path.join(REPO, "profiles", "queue_kernel_rolepair32_16384sys_dense64_pmc.md")
+def measured_traffic(profile="queue_kernel_rolepair32_16384sys_dense64_pmc.md"):
+    """HBM bytes per launch of a leg's kernel from the committed rocprofv3 counter summary of
+    the same command (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's
+    reading for a plain device copy of known size); by default the headline kernel's. None
+    when the summary is absent."""
+    path = os.path.join(REPO, "profiles", profile)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -610,9 +611,13 @@ def run_config4_leg(end_time, ns=4096):
             "accepted_steps_per_system": float(st[:, 0].mean()),
             "roofline": {"bound": "hbm", "achieved": m_bytes / wall / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": m_bytes / wall / 1e9 / HBM_PEAK_GBS,
+                         "traffic": (measured_traffic(
+                             "queue_kernel_wide256_config4_4096sys_trace_pmc.md")
+                             if ns == 4096 and end_time == 0.02 else None),
                          "note": "coupling-matrix bytes of the coupling solve (one pass over the "
                                  "system's M per BiCG iteration and one per stage) over the "
-                                 "launch; MALL hits can lift it past HBM"},
+                                 "launch; traffic: HBM bytes per launch measured by rocprofv3 "
+                                 "for this workload"},
             "fp64": {"achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS}}
 
