@@ -14,7 +14,7 @@ from bubbleensemble_implicitextension_amd.problem_solver import (LAYOUT_GROUP, L
                                                                   LAYOUT_SYSTEM_PAIR)
 from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
 
-from gpu_helpers import compare_solver_to_oracle, run_gpu, run_oracle
+from gpu_helpers import assert_bitwise, compare_solver_to_oracle, run_gpu, run_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -181,6 +181,35 @@ def test_config1_full_period_digests():
     assert not bad, f"blocks of {blk} systems differing from the oracle: {bad}"
     assert int(s.event_counts().sum()) == want["events_total"]
     assert int(s.stats()[:, 0].sum()) == want["accepted_total"]
+    s.close()
+
+
+@pytest.mark.parametrize("layout", [LAYOUT_ROLE_PAIR, LAYOUT_SYSTEM_PAIR])
+def test_config1_sample_three_periods_with_reset(oracle, layout):
+    """Twenty systems spread over config 1's grid (5 frequencies from 42.7 to 200 kHz x 4
+    amplitudes from 0.5 to 1.92 bar) over three consecutive periods — the second and third
+    start where the previous one ended, up to 6.1e4 accepted steps per period — with the
+    accessories reset after the first (the collapse-strength protocol) and Hermite dense
+    output at 64 points per period: state, time, accessories, counters and dense output bit for
+    bit like the oracle, on role pairs and on system pairs."""
+    sids = [f * 128 + p for f in (16, 48, 80, 112, 127) for p in (0, 40, 80, 120)]
+    ens = wl.config1().subset(sids)
+    ndo = 200
+    opts = {"DenseOutputMinimumTimeStep": 1.0 / 64, "DenseOutputInterpolation": 1}
+    print(f"case: config 1 sample of {ens.NS} systems, 3 periods, reset after 1, layout {layout}",
+          flush=True)
+    s = run_gpu(ens, periods=3, options=opts, ndo=ndo, reset_after=1, layout=layout)
+    p = run_oracle(oracle, ens, periods=3, options=opts, ndo=ndo, reset_after=1)
+    compare_solver_to_oracle(s, p)
+    assert s.stats()[:, 0].max() > 50000  # the later periods, not only the first from rest
+    n = int(p.DenseOutputIndex.max())
+    assert n == 3 * 65  # t = 0, 1/64, ..., 1 in every period
+    TLT = ens.NS * ens.N
+    assert_bitwise("DenseIndex", s.host_view(V.DenseIndex), p.DenseOutputIndex)
+    assert_bitwise("DenseTime", s.host_view(V.DenseTime)[: n * ens.NS],
+                   p.DenseOutputTimeInstances[: n * ens.NS])
+    assert_bitwise("DenseState", s.host_view(V.DenseState)[: n * TLT * 2],
+                   p.DenseOutputStates[: n * TLT * 2])
     s.close()
 
 
