@@ -1382,13 +1382,34 @@ struct PairLayout32 {
   __device__ __forceinline__ bool primal() const { return true; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 
-  __device__ __forceinline__ unsigned long long mask() const {
-    return grp ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+  // Every lane of a half gets its half's value of a per-half condition: lo_true / hi_true are
+  // the two halves' decisions, taken on the scalar unit from lane masks, and the resulting lane
+  // mask is the predicate itself (no vector instruction).
+  __device__ __forceinline__ static bool per_half(bool lo_true, bool hi_true) {
+    const unsigned long long m = (lo_true ? 0x00000000ffffffffull : 0ull) |
+                                 (hi_true ? 0xffffffff00000000ull : 0ull);
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+  }
+  // The two 32-bit halves of a lane mask as scalar words (the empty asm keeps the compiler from
+  // turning a test of the upper word into a 64-bit vector compare).
+  __device__ __forceinline__ static void words(unsigned long long m, unsigned* lo, unsigned* hi) {
+    unsigned l = (unsigned)m, h = (unsigned)(m >> 32);
+    asm("" : "+s"(l), "+s"(h));
+    *lo = l;
+    *hi = h;
   }
   // A half is either wholly active or wholly inactive, so the ballot over its 32 lanes is the
   // system's.
-  __device__ __forceinline__ bool all(bool p) const { return (__ballot(p) & mask()) == mask(); }
-  __device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask()) != 0ull; }
+  __device__ __forceinline__ bool all(bool p) const {
+    unsigned lo, hi;
+    words(__ballot(p), &lo, &hi);
+    return per_half(lo == 0xffffffffu, hi == 0xffffffffu);
+  }
+  __device__ __forceinline__ bool any(bool p) const {
+    unsigned lo, hi;
+    words(__ballot(p), &lo, &hi);
+    return per_half(lo != 0u, hi != 0u);
+  }
   // Halving-tree reductions of the half (Steppers.cuh:733-742) as butterflies: every lane of a
   // half ends with its system's value.
   __device__ __forceinline__ double sum(double v) const { return tree32(v, AddOp()); }
@@ -1411,9 +1432,10 @@ struct PairLayout32 {
   __device__ __forceinline__ bool res_test(double res, double tol) const {
     const unsigned long long numbers = __builtin_amdgcn_fcmp(res, res, 7);
     const unsigned long long failing = __builtin_amdgcn_fcmp(bex::fabs_d(res), tol, 11) & numbers;
-    const unsigned f = grp ? (unsigned)(failing >> 32) : (unsigned)failing;
-    const unsigned n = grp ? (unsigned)(numbers >> 32) : (unsigned)numbers;
-    return f == 0u && n != 0u;
+    unsigned flo, fhi, nlo, nhi;
+    words(failing, &flo, &fhi);
+    words(numbers, &nlo, &nhi);
+    return per_half(flo == 0u && nlo != 0u, fhi == 0u && nhi != 0u);
   }
 
   // A x = b with A = I + diag(rD) M diag(x1^2), b = xu + rD (M CPT), x0 = xu
