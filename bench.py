@@ -15,7 +15,7 @@ dense output stays in device memory (the PCIe-inclusive rate with each period's 
 copied to the host is reported beside it).
 
 Further legs, each skipped (and said so) when the wall budget would not hold it: the learned
-BiCG warm start; learned + batched (three solver objects launched together: the stiffest
+BiCG warm start; learned + batched (two solver objects launched together: the stiffest
 systems on the learned start, the next tier by measured cost one per wave, the rest two per
 wave) and the same split on the plain start (checked bit for bit against the plain leg); the
 BiCG residual 1e-13 workload (plain and learned); the second period of the headline workload
@@ -122,7 +122,7 @@ def cpu_baseline_child(limit):
     """cpu_baseline() in a child process (bench.py --leg cpu, which touches no GPU): its OpenMP
     worker threads end with the child instead of staying in this process, where their idle
     spinning competed with the host thread that launches and synchronises the GPU legs after
-    it (the three-part split's periods measured ~10 % slower behind an in-process baseline)."""
+    it (the split's periods measured ~10 % slower behind an in-process baseline)."""
     try:
         res = subprocess.run([sys.executable, os.path.abspath(__file__), "--leg", "cpu"],
                              capture_output=True, text=True, timeout=max(10.0, limit))
@@ -378,7 +378,7 @@ def wait_with_progress(pool, solvers, label, every=20.0, limit=None):
 
 
 def run_split(ens, network, heavy, medium, steps, warmup, end_time, cost):
-    """Times `steps` from-rest periods of the three-part split (batched.LearnedBatchedSolver)
+    """Times `steps` from-rest periods of the split (batched.LearnedBatchedSolver)
     with dense output; returns wall seconds, the results and the parts."""
     from bubbleensemble_implicitextension_amd import batched
 
@@ -398,8 +398,9 @@ def split_text(parts, learned):
     return (f"{len(parts['heavy'])} systems with the largest forcing amplitude / frequency on the "
             f"{start} start (one per wave), the next {len(parts['medium'])} by the cost measured "
             "in the plain leg's solve on the plain start (one per wave), the other "
-            f"{len(parts['light'])} on the plain start two per wave (system-pair layout); three "
-            "solver objects launched together, each step ends when all three are done")
+            f"{len(parts['light'])} on the plain start two per wave (system-pair layout); two "
+            "solver objects launched together (the one-per-wave tiers share one launch), each "
+            "step ends when both are done")
 
 
 def run_split_leg(steps, warmup, heavy, medium):
@@ -793,7 +794,7 @@ def main():
         }
         del learned
         log("learned leg done")
-    # Learned + batched: three parts launched together (batched.LearnedBatchedSolver). The
+    # Learned + batched: three tiers in two launches (batched.LearnedBatchedSolver). The
     # medium part is ranked by the cost each system measured in the plain leg's solve of this
     # period (its counters); in a run of consecutive periods, the previous period's counters.
     measured = batched.measured_cost(stats, N)

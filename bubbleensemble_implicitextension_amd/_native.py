@@ -82,6 +82,8 @@ _SIGNATURES = {
     "bex_checkpoint_restore": ([ctypes.c_void_p], ctypes.c_int),
     "bex_set_launch_order": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int],
                              ctypes.c_int),
+    "bex_set_warm_start_systems": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte),
+                                    ctypes.c_int], ctypes.c_int),
     "bex_persistent_grid": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                              ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "bex_request_stop": ([ctypes.c_void_p], ctypes.c_int),

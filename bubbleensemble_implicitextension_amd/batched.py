@@ -69,20 +69,21 @@ def split_systems(ens, heavy, medium, cost):
 
 
 class LearnedBatchedSolver:
-    """Learned + batched: the ensemble split into parts that run as separate solver objects
-    (each on its own stream), launched together, costliest part first:
+    """Learned + batched: the ensemble split into three tiers that run as two solver objects
+    (each on its own stream), launched together, costliest first:
 
     * heavy: the `heavy` systems with the largest cost_proxy (the stiff corner of the forcing
       grid, near the Keller-Miksis Mach-1 singularity) on the learned BiCG start, one system per
       wave (role pair). There the learned start cuts the BiCG iterations and shortens the
       systems that bound the launch; elsewhere it saves none and costs its network evaluation.
     * medium: the next `medium` systems by `cost` on the plain start, one system per wave (role
-      pair: the shortest time per system).
+      pair: the shortest time per system), in the same launch as the heavy ones (the learned
+      kernel with the learned start switched off per system).
     * light: the rest on the plain start on the system-pair layout (two systems per wave: the
       right-hand side, reductions and controller run once for both), the highest throughput.
 
-    cost ranks the systems for the medium part (default cost_proxy(ens); the counters of a
-    previous solve give measured_cost). Each part's persistent launch hands its systems out
+    cost ranks the systems for the medium tier (default cost_proxy(ens); the counters of a
+    previous solve give measured_cost). Each launch is persistent and hands its systems out
     costliest first; the heavy and medium waves issue at the top priority. With network=None
     the heavy systems take the plain start too (a plain batched split, bit-identical to the
     plain solve system by system).
@@ -97,10 +98,11 @@ class LearnedBatchedSolver:
         self.heavy, self.medium, self.light = split_systems(ens, heavy, medium, cost)
         self.NS, self.N = ens.NS, ens.N
         self.parts = []
-        plan = [(self.heavy, network, LAYOUT_AUTO, True)]
-        if self.medium.size > 0:
-            plan.append((self.medium, None, LAYOUT_AUTO, True))
-        plan.append((self.light, None, light_layout, False))
+        # The heavy and medium systems share one launch (one wave each, all started at once, so
+        # no two of these top-priority waves meet on a SIMD); the light systems are the other.
+        one_per_wave = np.sort(np.concatenate([self.heavy, self.medium]))
+        plan = [(one_per_wave, network, LAYOUT_AUTO, True),
+                (self.light, None, light_layout, False)]
         for sids, net, layout, urgent in plan:
             sub = ens.subset(sids)
             s = collapse.make_solver(sub, NDO=ndo)
@@ -122,6 +124,10 @@ class LearnedBatchedSolver:
             if net is not None:
                 s.set_warm_start_network(net)
                 s.SolverOption(O.WarmStart, 1)
+                if sids.size != self.heavy.size:
+                    # only the heavy systems take the learned start, the medium ones the plain
+                    # start (bit-identical to the plain solve)
+                    s.set_warm_start_systems(np.isin(sids, self.heavy))
             s.set_launch_order(hand_out_order(cost[sids], 2 if layout == LAYOUT_SYSTEM_PAIR else 1))
             s.SynchroniseFromHostToDevice(V.All)
             self.parts.append((sids, s))

@@ -77,6 +77,7 @@ struct bex_solver {
   int PersistentWorkgroups = 0;  // > 0: cap on the persistent launch's workgroups
   unsigned long long* d_queue = nullptr;  // its counter
   int* d_order = nullptr;                 // hand-out order of the systems (nullptr: index order)
+  unsigned char* d_warm_systems = nullptr;  // [NS] which systems take the learned start
   int cu_count = 0;
   int PriorityStepThreshold = 3000;
   long long MaxStepAttempts = 0;  // 0: no cap
@@ -285,6 +286,7 @@ void fill_args(const bex_solver* s, BexKernelArgs* ap) {
   a.BiCGMaxIterations = s->BiCGMaxIterations;
   a.PriorityStepThreshold = s->PriorityStepThreshold > 0 ? s->PriorityStepThreshold : 0x3fffffff;
   a.WarmStart = s->d_net;
+  a.WarmStartSystems = s->d_warm_systems;
   a.Samples = s->d_samples;
   a.SampleCount = s->d_sample_count;
   a.SampleCapacity = s->SampleCapacity;
@@ -655,6 +657,7 @@ int bex_destroy(bex_solver* s) {
   if (s->d_stats) hipFree(s->d_stats);
   if (s->d_queue) hipFree(s->d_queue);
   if (s->d_order) hipFree(s->d_order);
+  if (s->d_warm_systems) hipFree(s->d_warm_systems);
   if (s->h_timeline) hipHostFree(s->h_timeline);
   if (s->h_stop) hipHostFree(s->h_stop);
   if (s->d_DenseAux) hipFree(s->d_DenseAux);
@@ -1123,6 +1126,26 @@ int bex_solve(bex_solver* s) {
   dim3 grid((nsys + kc.systems_per_block - 1) / kc.systems_per_block), block(kc.block);
   hipLaunchKernelGGL(kc.fn, grid, block, kc.lds, s->stream, a, (long long)nsys);
   BEX_HIP(s, hipGetLastError());
+  return BEX_OK;
+}
+
+int bex_set_warm_start_systems(bex_solver* s, const unsigned char* flags, int n) {
+  if (!s) return BEX_E_ARGUMENT;
+  BEX_HIP(s, hipSetDevice(s->device));
+  if (!flags || n == 0) {
+    if (s->d_warm_systems) {
+      BEX_HIP(s, hipStreamSynchronize(s->stream));
+      BEX_HIP(s, hipFree(s->d_warm_systems));
+      s->d_warm_systems = nullptr;
+    }
+    return BEX_OK;
+  }
+  if (n != s->d.NS)
+    return fail(s, BEX_E_ARGUMENT, "bex_set_warm_start_systems: one flag per system is needed");
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
+  if (!s->d_warm_systems) BEX_HIP(s, hipMalloc((void**)&s->d_warm_systems, (size_t)n));
+  BEX_HIP(s, hipMemcpyAsync(s->d_warm_systems, flags, (size_t)n, hipMemcpyHostToDevice, s->stream));
+  BEX_HIP(s, hipStreamSynchronize(s->stream));
   return BEX_OK;
 }
 

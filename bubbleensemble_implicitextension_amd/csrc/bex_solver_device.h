@@ -1108,6 +1108,7 @@ struct RolePairLayout32 {
 
   int lane, sid_;
   bool primal_;
+  bool learned_;    // learned kernel: this system takes the learned start (wave-uniform)
   double mrow[32];  // row i of M (primal) or of M^T (shadow)
   Sys* sys;
   int* stat;
@@ -1141,6 +1142,9 @@ struct RolePairLayout32 {
     int s = -1;
     if (item < count) s = order ? order[item] : a.first_system + (int)item;
     sid_ = s;
+    const int take = (MODE == kLearned && s >= 0 && s < a.NS &&
+                      (!a.WarmStartSystems || a.WarmStartSystems[s] != 0)) ? 1 : 0;
+    learned_ = __builtin_amdgcn_readfirstlane(take) != 0;
     if (lane < BEX_NSTAT) stat[lane] = 0;
     const int i = lane & 31;
     if (s >= 0 && s < a.NS) {
@@ -1246,7 +1250,7 @@ struct RolePairLayout32 {
     double lo, up;
     double wf[BEX_MLP_IN];
     double bl = 0.0;
-    const bool need_features = (MODE == kLearned) || (MODE == kSampling && sample_now);
+    const bool need_features = (MODE == kLearned && learned_) || (MODE == kSampling && sample_now);
     if (need_features) {
       halves(matvec(x1sq), &lo, &up);  // sum_j M_ij x1_j^2 of the lane's unit, both halves
       feat(lo, wf);
@@ -1256,7 +1260,7 @@ struct RolePairLayout32 {
     double r0 = residual(b, xu, x1sq, rD);
     bool learned_start = false;  // wave-uniform
     double mu = 0.0;             // max |r0| of the plain start (learned kernel)
-    if (MODE == kLearned) {
+    if (MODE == kLearned && learned_) {
       // The learned start is used only when its initial residual is smaller (max norm) than
       // the plain start's, so BiCG never starts further from the solution than the reference.
       const float y = mlp_factor(net, wf, lane);

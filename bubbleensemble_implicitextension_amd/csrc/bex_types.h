@@ -77,6 +77,8 @@ struct BexKernelArgs {
 
   // BiCG warm start and training-sample collection (no reference counterpart).
   const BexMlpDevice* WarmStart;     // network in the solver's operand layout (learned kernel)
+  const unsigned char* WarmStartSystems;  // [NS] learned kernel: 0 = plain start for that
+                                          // system; nullptr: every system takes the learned start
   float* Samples;                    // [SampleCapacity][BEX_SAMPLE_FLOATS]
   unsigned long long* SampleCount;   // rows reserved so far (may exceed the capacity)
   long long SampleCapacity;

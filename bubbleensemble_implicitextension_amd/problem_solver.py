@@ -261,6 +261,17 @@ class ProblemSolver:
         self._check(self._lib.bex_persistent_grid(self._h, ctypes.byref(wg), ctypes.byref(w)))
         return wg.value, w.value
 
+    def set_warm_start_systems(self, flags):
+        """Which systems take the learned BiCG start when WarmStart is on: one flag per system
+        (False: the plain start, results bit-identical to the plain solve); None: every
+        system."""
+        if flags is None:
+            self._check(self._lib.bex_set_warm_start_systems(self._h, None, 0))
+            return
+        f = np.ascontiguousarray(np.asarray(flags, dtype=bool), dtype=np.uint8)
+        self._check(self._lib.bex_set_warm_start_systems(
+            self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), int(f.size)))
+
     def set_launch_order(self, order):
         """Order in which Solve()'s persistent launch hands out the systems: every system
         index once, longest expected run first; None returns to index order. Results do not

@@ -606,16 +606,24 @@ class ProblemSolver {
                                static_cast<int>(order.size())),
           "SetLaunchOrder");
   }
+
+  // Which systems take the learned BiCG start when WarmStart is on (no reference counterpart):
+  // one flag per system, 0 = plain start; an empty vector: every system.
+  void SetWarmStartSystems(const std::vector<unsigned char>& flags) {
+    Check(bex_set_warm_start_systems(handle_, flags.empty() ? nullptr : flags.data(),
+                                     static_cast<int>(flags.size())),
+          "SetWarmStartSystems");
+  }
 };
 
-// Split solver (no reference counterpart): one ensemble held by up to three solver objects that
-// run together, each a persistent launch on its own stream — the `heavy` systems one per wave
-// (on the learned BiCG start when a network is given), the `medium` systems one per wave on the
-// plain start, the others two per wave (system-pair layout, 32 units; one per wave otherwise).
-// The heavy and medium waves issue at the top priority. The member functions are
-// ProblemSolver's, with the system indices of the whole ensemble; without a network every
-// result equals ProblemSolver's bit for bit. Print is not provided (each part is an ordinary
-// solver object; Part(k) gives its handle for the C calls).
+// Split solver (no reference counterpart): one ensemble held by two solver objects that run
+// together, each a persistent launch on its own stream — the `heavy` systems (on the learned
+// BiCG start when a network is given) and the `medium` systems (plain start) one per wave in
+// one launch, at the top issue priority, and the others two per wave (system-pair layout,
+// 32 units; one per wave otherwise). The member functions are ProblemSolver's, with the system
+// indices of the whole ensemble; the plain-start systems' results equal ProblemSolver's bit for
+// bit. Print is not provided (each part is an ordinary solver object; Part(k) gives its handle
+// for the C calls).
 template <int NS, int UPS, int UD, int TPB, int SPB, int NC, int CBW, int CCI, int NUP, int NSP,
           int NGP, int NiGP, int NUA, int NiUA, int NSA, int NiSA, int NE, int NDO,
           Algorithms Algorithm, class Precision>
@@ -652,20 +660,22 @@ class SplitProblemSolver {
   SplitProblemSolver(int device, const std::vector<int>& heavy, const std::vector<int>& medium,
                      const float* network = nullptr)
       : part_of_(NS, 2), local_of_(NS, 0) {
-    std::vector<std::vector<int>> groups(3);
+    // tier: 0 heavy, 1 medium, 2 light; part: 0 heavy and medium, 1 light
+    std::vector<int> tier(NS, 2);
     for (int k = 0; k < 2; k++) {
       for (int sid : (k == 0 ? heavy : medium)) {
-        if (sid < 0 || sid >= NS || part_of_[sid] != 2) {
+        if (sid < 0 || sid >= NS || tier[sid] != 2) {
           std::cerr << "ERROR: SplitProblemSolver: system " << sid
                     << " is out of range or listed twice" << std::endl;
           std::exit(EXIT_FAILURE);
         }
-        part_of_[sid] = k;
+        tier[sid] = k;
       }
     }
-    for (int sid = 0; sid < NS; sid++) groups[part_of_[sid]].push_back(sid);
-    std::vector<int> slot(3, -1);
-    for (int k = 0; k < 3; k++) {
+    std::vector<std::vector<int>> groups(2);
+    for (int sid = 0; sid < NS; sid++) groups[tier[sid] < 2 ? 0 : 1].push_back(sid);
+    std::vector<int> slot(2, -1);
+    for (int k = 0; k < 2; k++) {
       if (groups[k].empty()) continue;
       Piece part;
       part.sids = groups[k];
@@ -676,22 +686,25 @@ class SplitProblemSolver {
         std::cerr << "       " << bex_last_error(nullptr) << std::endl;
         std::exit(EXIT_FAILURE);
       }
-      const bool urgent = k < 2;
-      if (urgent) {
+      if (k == 0) {
         Check(bex_solver_option(part.h, BEX_PriorityStepThreshold, 1.0), "SplitProblemSolver", part.h);
+        if (network && !heavy.empty()) {
+          Check(bex_set_warm_start_network(part.h, network), "SplitProblemSolver", part.h);
+          Check(bex_solver_option(part.h, BEX_WarmStart, 1.0), "SplitProblemSolver", part.h);
+          std::vector<unsigned char> learned(part.sids.size());
+          for (size_t i = 0; i < part.sids.size(); i++) learned[i] = tier[part.sids[i]] == 0;
+          Check(bex_set_warm_start_systems(part.h, learned.data(), (int)learned.size()),
+                "SplitProblemSolver", part.h);
+        }
       } else if (UPS == 32) {
         Check(bex_solver_option(part.h, BEX_KernelLayout, BEX_LAYOUT_SYSTEM_PAIR),
               "SplitProblemSolver", part.h);
-      }
-      if (k == 0 && network) {
-        Check(bex_set_warm_start_network(part.h, network), "SplitProblemSolver", part.h);
-        Check(bex_solver_option(part.h, BEX_WarmStart, 1.0), "SplitProblemSolver", part.h);
       }
       slot[k] = (int)parts_.size();
       for (int i = 0; i < (int)part.sids.size(); i++) local_of_[part.sids[i]] = i;
       parts_.push_back(part);
     }
-    for (int sid = 0; sid < NS; sid++) part_of_[sid] = slot[part_of_[sid]];
+    for (int sid = 0; sid < NS; sid++) part_of_[sid] = slot[tier[sid] < 2 ? 0 : 1];
   }
   ~SplitProblemSolver() {
     for (Piece& p : parts_)

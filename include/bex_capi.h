@@ -206,6 +206,12 @@ int bex_clear_stop(bex_solver* s);
  * systems of similar cost should be neighbours. Results do not depend on it. */
 int bex_set_launch_order(bex_solver* s, const int* order, int n);
 
+/* Which systems take the learned BiCG start when WarmStart is on (no reference counterpart):
+ * flags[0..n-1], n = NS, 0 = the plain start for that system (its results are then those of
+ * the plain solve bit for bit); flags = NULL or n = 0: every system takes it. Lets one launch
+ * run learned and plain systems side by side. */
+int bex_set_warm_start_systems(bex_solver* s, const unsigned char* flags, int n);
+
 /* Workgroups of the persistent launch that the next Solve() would start (as many as fit on the
  * device, or PersistentWorkgroups when that is smaller, and no more than the systems need; 0
  * without systems to solve), and the waves each has; BEX_E_UNSUPPORTED with PersistentLaunch 0

@@ -150,6 +150,12 @@ def test_error_conventions():
             s.set_launch_order(bad)
     s.set_launch_order([3, 2, 1, 0])
     s.set_launch_order(None)
+    # The learned-start flags need one entry per system.
+    for bad in ([1, 0, 1], [1, 0, 1, 0, 1]):
+        with pytest.raises(BexError, match="one flag per system"):
+            s.set_warm_start_systems(bad)
+    s.set_warm_start_systems([1, 0, 0, 1])
+    s.set_warm_start_systems(None)
     s.close()
 
 
@@ -257,15 +263,15 @@ def test_cpp_driver_matches_python_protocol(tmp_path):
     np.testing.assert_allclose(a[:, 3:], b[:, 3:], rtol=2e-3, atol=0)
     assert (a[:, 3:] > 0).mean() > 0.5  # collapses were recorded
 
-    # The same scan on the C++ split solver (SplitProblemSolver: 4 systems one per wave, the next
-    # 8 one per wave, the other 52 two per wave, plain start): the same result file, byte for
-    # byte.
+    # The same scan on the C++ split solver (SplitProblemSolver: 4 + 8 systems one per wave in
+    # one launch, the other 52 two per wave in another, plain start): the same result file, byte
+    # for byte.
     split_out = tmp_path / "split.txt"
     print("C++ driver on the split solver: 64 systems x 36 periods", flush=True)
     r = subprocess.run([APP, RADII, COUPLING, str(split_out), str(pa_lo), str(pa_hi), str(f_lo),
                         str(f_hi), "4", "8"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "split solver: 3 parts" in r.stdout
+    assert "split solver: 2 parts" in r.stdout
     assert split_out.read_text() == cpp_out.read_text()
 
 

@@ -174,9 +174,11 @@ def test_learned_warm_start_ablation():
 
 def test_learned_batched_split():
     """Learned + batched: the costliest systems by the amplitude/frequency proxy run with the
-    learned start, the others with the plain start, both launched together; the plain-start
-    systems are bit-identical to a plain solve of the whole ensemble and the learned-start
-    systems stay within the learned start's stated error."""
+    learned start and the next 24 with the plain start in the same launch (the learned kernel,
+    the learned start switched off per system), the others with the plain start two per wave,
+    both launches together; every plain-start system is bit-identical to a plain solve of the
+    whole ensemble and the learned-start systems stay within the learned start's stated
+    error."""
     from bubbleensemble_implicitextension_amd import collapse
     from bubbleensemble_implicitextension_amd import warm_start as wsm
     from bubbleensemble_implicitextension_amd import workloads as wl
@@ -191,11 +193,13 @@ def test_learned_batched_split():
     ens = full.subset(np.sort(np.concatenate([spread, corner])))
     net = np.load(wsm.DEFAULT_NETWORK)
     print(f"case: split of {ens.NS} systems, 16 on the learned start, tau in [0, 1]", flush=True)
-    lb = wsm.LearnedBatchedSolver(ens, net, heavy=16, end_time=1.0)
+    lb = wsm.LearnedBatchedSolver(ens, net, heavy=16, medium=24, end_time=1.0)
+    assert len(lb.parts) == 2 and lb.medium.size == 24
     lb.Solve()
     lb.synchronise()
     got = lb.results()
-    heavy, light = lb.heavy, lb.light
+    heavy = lb.heavy
+    light = np.sort(np.concatenate([lb.medium, lb.light]))  # every plain-start system
     lb.close()
     want_heavy = np.sort(np.argsort(-wsm.cost_proxy(ens), kind="stable")[:16])
     np.testing.assert_array_equal(heavy, want_heavy)
