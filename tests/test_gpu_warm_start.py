@@ -228,3 +228,47 @@ def test_learned_batched_split():
     assert it_learned < it_plain, (it_learned, it_plain)
     np.testing.assert_array_equal(got["events"][heavy], ev[heavy])
     s.close()
+
+
+def test_per_system_learned_start_flags(oracle):
+    """bex_set_warm_start_systems: in one launch of the learned kernel, the systems whose flag is
+    off take the plain start and match the plain oracle bit for bit (state, time, accessories,
+    counters), the flagged ones take the learned start (they step and stay finite); with every
+    flag off the whole launch is the plain solve."""
+    import sys
+    import os
+    from bubbleensemble_implicitextension_amd import batched
+    from bubbleensemble_implicitextension_amd import workloads as wl
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfSolverOptions as O
+    from bubbleensemble_implicitextension_amd.problem_solver import ListOfVariables as V
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from gpu_helpers import assert_bitwise, run_gpu, run_oracle
+
+    full = wl.config1()
+    corner = np.argsort(-batched.cost_proxy(full), kind="stable")[:4]
+    ens = full.subset(np.sort(np.concatenate([corner, np.arange(300, 16384, 3000)])))
+    net = np.load(batched.DEFAULT_NETWORK)
+    flags = np.zeros(ens.NS, dtype=bool)
+    flags[::2] = True
+    p = run_oracle(oracle, ens, periods=1, end_time=0.3)
+    for f in (flags, np.zeros(ens.NS, dtype=bool)):
+        print(f"case: {ens.NS} systems, learned-start flags {f.astype(int).tolist()}", flush=True)
+
+        def setup(s, f=f):
+            s.set_warm_start_network(net)
+            s.SolverOption(O.WarmStart, 1)
+            s.set_warm_start_systems(f)
+
+        s = run_gpu(ens, periods=1, end_time=0.3, setup=setup)
+        off = ~f
+        N = ens.N
+        for var, want in ((V.ActualState, p.ActualState), (V.UnitAccessories, p.UnitAccessories)):
+            got = s.host_view(var).reshape(-1, ens.NS, N)
+            assert_bitwise(var.name, got[:, off], want.reshape(-1, ens.NS, N)[:, off])
+        assert_bitwise("ActualTime", s.host_view(V.ActualTime)[off], p.ActualTime[off])
+        assert_bitwise("stats", s.stats()[off], p.stats_table()[off])
+        if f.any():
+            assert np.all(np.isfinite(s.host_view(V.ActualState).reshape(-1, ens.NS, N)[:, f]))
+            assert np.all(s.stats()[f, 0] > 0)
+        s.close()
